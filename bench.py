@@ -1,0 +1,229 @@
+"""Benchmark of the FFT hot path on one MI355X.
+
+Workload (one step): batched C2C FP32 forward + backward round trip, N = 4096,
+batch 16384 rows (512 MiB per direction), re/im uniform[-1,1) from seed 1, resident in
+HBM before the timed region.  Metric: whole-step algorithmic HBM throughput in GB/s
+(16 bytes per complex value per direction: one read + one write), beside:
+
+  * roofline of the dominant kernel (k_fft_rows, N=4096): algorithmic bytes per launch
+    / mean launch duration from HIP events on the launch stream, against 8 TB/s;
+  * the per-call form (one launch per transform, same kernels, same stream) and the
+    batched/per-call throughput ratio;
+  * p50/p99 step latency, and p50/p99 latency of a single N = 65536 transform;
+  * relative L2 error of sampled rows against the FP64 oracle;
+  * a CPU baseline: the oracle's FP32 restatement of the reference algorithm, one core.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N = 4096
+BATCH = 16384
+SEED = 1
+METRIC = "batched C2C fp32 GB/s & % HBM peak vs per-call baseline; p50/p99 latency"
+
+
+def percentile(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * p / 100.0
+    lo, hi = int(math.floor(k)), int(math.ceil(k))
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def load_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter
+    summary (profiles/traffic_k_fft_rows_4096x16384.json), if present."""
+    path = os.path.join(ROOT, "profiles", "traffic_k_fft_rows_4096x16384.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
+
+
+def cpu_baseline(rows: int, reps: int):
+    import numpy as np
+    import oracle
+    x = oracle.uniform_complex(rows, N, SEED)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        y = oracle.fft32(x, oracle.FORWARD)
+        oracle.fft32(y, oracle.BACKWARD)
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[len(times) // 2]
+    bytes_ = 2 * 16.0 * N * rows
+    return {"value": round(bytes_ / t / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"oracle FP32 restatement of FFT_OpenCL_Contiguous (radix-16 Stockham passes), "
+                      f"{rows} rows x N={N}, forward + backward, median of {reps} runs, 1 thread"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--percall-rows", type=int, default=BATCH)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              f"(the workload shards by batch row with no exchange; see the sharding note of the design)",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    import estl_amd
+    import oracle
+
+    estl_amd.init(0)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(dev)
+    x = torch.empty(BATCH, N, dtype=torch.complex64, device=dev)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    with torch.cuda.stream(stream):
+        estl_amd.fill_uniform(x, SEED, stream=stream)
+    stream.synchronize()
+
+    fwd = estl_amd.Plan(N, BATCH, estl_amd.FORWARD)
+    bwd = estl_amd.Plan(N, BATCH, estl_amd.BACKWARD)
+    plan_desc = fwd.describe()
+
+    def step():
+        fwd.execute(x, y, stream)
+        bwd.execute(y, z, stream)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+
+    # timed region: exactly K steps, synchronised on both sides
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    stream.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    step_bytes = 2 * 16.0 * N * BATCH
+    value = step_bytes / (wall / args.steps) / 1e9
+
+    # per-launch durations with events on the launch stream (separate pass)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    for e in ev:
+        e[0].record(stream)
+        fwd.execute(x, y, stream)
+        e[1].record(stream)
+        bwd.execute(y, z, stream)
+        e[2].record(stream)
+    stream.synchronize()
+    fwd_ms = [e[0].elapsed_time(e[1]) for e in ev]
+    bwd_ms = [e[1].elapsed_time(e[2]) for e in ev]
+    step_ms = [e[0].elapsed_time(e[2]) for e in ev]
+    launch_bytes = 16.0 * N * BATCH
+    fwd_mean = sum(fwd_ms) / len(fwd_ms)
+    bwd_mean = sum(bwd_ms) / len(bwd_ms)
+    achieved = launch_bytes / (fwd_mean * 1e-3) / 1e9
+
+    # accuracy of the timed result: sampled rows against the FP64 oracle, round trip
+    rows = [0, 1, 1234, 8191, BATCH - 1]
+    xs = x[rows].cpu().numpy()
+    err_fwd = max(oracle.rel_l2(y[r].cpu().numpy(), oracle.fft64(xs[i])) for i, r in enumerate(rows))
+    rt = (torch.linalg.vector_norm((z / N - x).to(torch.complex128)) /
+          torch.linalg.vector_norm(x.to(torch.complex128))).item()
+
+    # per-call form: one launch per transform, sequential, same stream, device-resident
+    prow = min(args.percall_rows, BATCH)
+    f1 = estl_amd.Plan(N, 1, estl_amd.FORWARD)
+    b1 = estl_amd.Plan(N, 1, estl_amd.BACKWARD)
+    xb, yb, zb = x.data_ptr(), y.data_ptr(), z.data_ptr()
+    rb = N * 8
+
+    def percall():
+        for r in range(prow):
+            f1.execute(xb + r * rb, yb + r * rb, stream)
+        for r in range(prow):
+            b1.execute(yb + r * rb, zb + r * rb, stream)
+
+    percall()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    percall()
+    stream.synchronize()
+    pc_wall = time.perf_counter() - t0
+    percall_gbs = 2 * 16.0 * N * prow / pc_wall / 1e9
+
+    # single-transform latency, N = 65536 (batch 1), p50/p99 over 1000 launches
+    n1 = 1 << 16
+    a1 = torch.empty(1, n1, dtype=torch.complex64, device=dev)
+    o1 = torch.empty_like(a1)
+    estl_amd.fill_uniform(a1, 0, stream=stream)
+    p1 = estl_amd.Plan(n1, 1, estl_amd.FORWARD)
+    for _ in range(20):
+        p1.execute(a1, o1, stream)
+    lat = []
+    for _ in range(1000):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        p1.execute(a1, o1, stream)
+        e.record(stream)
+        lat.append((s, e))
+    stream.synchronize()
+    lat_us = [1e3 * s.elapsed_time(e) for s, e in lat]
+    err_c1 = oracle.rel_l2(o1.cpu().numpy()[0], oracle.fft64(a1.cpu().numpy()[0]))
+
+    cpu = None if args.no_cpu else cpu_baseline(BATCH, 3)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched C2C fp32 forward+backward round trip, N=4096, batch 16384, "
+                               "uniform[-1,1) seed 1, HBM-resident",
+                   "n": N, "batch": BATCH, "seed": SEED, "plan": plan_desc},
+        "hbm_frac": round(value / HBM_PEAK_GBS, 4),
+        "gflops_5nlog2n": round(2 * 5.0 * N * math.log2(N) * BATCH / (wall / args.steps) / 1e9, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(),
+                     "kernel": "k_fft_rows<12,-1,1> (forward, N=4096)",
+                     "launch_ms_mean": round(fwd_mean, 5), "backward_launch_ms_mean": round(bwd_mean, 5)},
+        "per_call": {"value": round(percall_gbs, 2), "unit": "GB/s", "rows": prow,
+                     "launches": 2 * prow, "batched_over_per_call": round(value / percall_gbs, 1)},
+        "latency_us": {"step_p50": round(1e3 * percentile(step_ms, 50), 2),
+                       "step_p99": round(1e3 * percentile(step_ms, 99), 2),
+                       "n65536_batch1_p50": round(percentile(lat_us, 50), 2),
+                       "n65536_batch1_p99": round(percentile(lat_us, 99), 2)},
+        "rel_l2_vs_fp64": {"forward_sampled_rows": err_fwd, "round_trip": rt, "n65536": err_c1,
+                           "bound": 1e-6 * math.log2(N)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

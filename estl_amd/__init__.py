@@ -1,0 +1,187 @@
+"""estl_amd -- Python binding of the gfx950 FFT hot path (libestl_hip.so).
+
+The compute path is the HIP library behind the C ABI of ``include/estl_hip.h``;
+this module only loads it with ctypes and passes device pointers (from torch
+tensors, which serve as device-memory and stream plumbing).  There is no CPU
+fallback: if the library is missing, ``lib()`` raises.
+
+The Python names mirror the reference's FFT interface (FFT.h:19-52):
+``FFT_HIP(n, forward=True, batch_count=1)`` with ``execute()`` on host arrays, and
+``Plan`` for the asynchronous device-resident form.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libestl_hip.so")
+
+FORWARD = -1
+BACKWARD = +1
+FP32 = 0
+FP16 = 1
+FLAG_NORMALIZE = 0x1
+
+OK = 0
+ERR_INVALID = -1
+ERR_HIP = -2
+ERR_NOMEM = -3
+ERR_UNSUPPORTED = -4
+ERR_BUSY = -5
+
+_lib = None
+
+
+class EstlError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"estl error {code}: {msg}")
+        self.code = code
+
+
+def _sig(L, name, restype, argtypes):
+    f = getattr(L, name)
+    f.restype = restype
+    f.argtypes = argtypes
+    return f
+
+
+def lib():
+    """Load libestl_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m estl_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, cl, ci, cu = ctypes.c_void_p, ctypes.c_long, ctypes.c_int, ctypes.c_uint
+    ll, u64 = ctypes.c_longlong, ctypes.c_uint64
+    _sig(L, "estl_last_error", ctypes.c_char_p, [])
+    _sig(L, "estl_version", ci, [])
+    _sig(L, "estl_init", ci, [ci])
+    _sig(L, "estl_plan_create", ci, [ctypes.POINTER(vp), cl, cl, ci, ci, cu])
+    _sig(L, "estl_plan_create_split", ci, [ctypes.POINTER(vp), cl, cl, ci, ci, cu, ci, ctypes.POINTER(ci)])
+    _sig(L, "estl_plan_destroy", ci, [vp])
+    _sig(L, "estl_execute_device", ci, [vp, vp, vp, vp])
+    _sig(L, "estl_execute_host", ci, [vp, vp, vp])
+    _sig(L, "estl_max_batch", ll, [cl, ci, ll])
+    _sig(L, "estl_plan_workspace_bytes", ll, [vp])
+    _sig(L, "estl_plan_describe", ci, [vp, ctypes.c_char_p, ci])
+    _sig(L, "estl_plan_launches", ci, [vp])
+    _sig(L, "estl_fill_uniform", ci, [vp, cl, u64, u64, vp])
+    _lib = L
+    return L
+
+
+def check(rc: int) -> int:
+    if rc < 0:
+        raise EstlError(rc, lib().estl_last_error().decode())
+    return rc
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def init(device: int = 0) -> None:
+    check(lib().estl_init(device))
+
+
+class Plan:
+    """Device plan for `batch` rows of length n (C ABI estl_plan_create)."""
+
+    def __init__(self, n: int, batch: int = 1, direction: int = FORWARD, precision: int = FP32,
+                 flags: int = 0, split: list[int] | None = None):
+        self.n, self.batch, self.direction, self.precision, self.flags = n, batch, direction, precision, flags
+        self._h = ctypes.c_void_p()
+        h = ctypes.c_void_p()
+        if split:
+            arr = (ctypes.c_int * len(split))(*split)
+            check(lib().estl_plan_create_split(ctypes.byref(h), n, batch, direction, precision, flags,
+                                               len(split), arr))
+        else:
+            check(lib().estl_plan_create(ctypes.byref(h), n, batch, direction, precision, flags))
+        self._h = h
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+    def execute(self, d_in, d_out, stream=None) -> None:
+        """Run on device tensors (or raw device pointers) on `stream` (torch stream)."""
+        pin = d_in if isinstance(d_in, int) else d_in.data_ptr()
+        pout = d_out if isinstance(d_out, int) else d_out.data_ptr()
+        check(lib().estl_execute_device(self._h, pin, pout, _stream_ptr(stream)))
+
+    def execute_host(self, h_in, h_out) -> None:
+        """Synchronous host-to-host execute (numpy arrays, complex64)."""
+        check(lib().estl_execute_host(self._h, h_in.ctypes.data, h_out.ctypes.data))
+
+    def describe(self) -> dict:
+        buf = ctypes.create_string_buffer(4096)
+        check(lib().estl_plan_describe(self._h, buf, 4096))
+        return json.loads(buf.value.decode())
+
+    def launches(self) -> int:
+        return check(lib().estl_plan_launches(self._h))
+
+    def workspace_bytes(self) -> int:
+        return check(lib().estl_plan_workspace_bytes(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().estl_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fill_uniform(t, seed: int, offset: int = 0, stream=None) -> None:
+    """Fill a float32/complex64 device tensor with the seeded uniform[-1,1) stream."""
+    import torch
+    count = t.numel() * (2 if t.dtype == torch.complex64 else 1)
+    check(lib().estl_fill_uniform(t.data_ptr(), count, seed, offset, _stream_ptr(stream)))
+
+
+def max_batch(n: int, precision: int = FP32, budget_bytes: int = 0) -> int:
+    return check(lib().estl_max_batch(n, precision, budget_bytes))
+
+
+class FFT_HIP:
+    """Mirror of the reference's backend interface (FFT.h:19-36, FFT_FFTW3.h:16-34):
+    ``FFT_HIP(n, forward, batch_count)``, ``set_data``/``get_data`` by (index, batch
+    index) and a synchronous ``execute()`` over host arrays."""
+
+    @staticmethod
+    def get_name() -> str:
+        return "HIP FFT (gfx950)"
+
+    @staticmethod
+    def get_max_batch_size(n: int) -> int:
+        return max_batch(n)
+
+    def __init__(self, n: int, forward: bool = True, batch_count: int = 1):
+        import numpy as np
+        self.size, self.batch_count = n, batch_count
+        self.plan = Plan(n, batch_count, FORWARD if forward else BACKWARD)
+        self.data_in = np.zeros(n * batch_count, dtype=np.complex64)
+        self.data_out = np.zeros(n * batch_count, dtype=np.complex64)
+
+    def set_data(self, index: int, value: complex, batch_index: int = 0) -> None:
+        self.data_in[batch_index * self.size + index] = value
+
+    def get_data(self, index: int, batch_index: int = 0) -> complex:
+        return complex(self.data_out[batch_index * self.size + index])
+
+    def execute(self) -> None:
+        self.plan.execute_host(self.data_in, self.data_out)

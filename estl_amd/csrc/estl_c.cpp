@@ -1,0 +1,408 @@
+// estl_c.cpp -- host side of the C ABI declared in include/estl_hip.h: plans, the
+// per-device twiddle cache, execution and status reporting.
+//
+// The planner is the counterpart of FFT_OpenCL_Contiguous's constructor
+// (FFT_OpenCL_Contiguous.h:46-71): where the reference emits one OpenCL kernel string
+// per radix-16 pass and compiles a program per FFT object, a plan here picks among
+// precompiled gfx950 kernels:
+//   * rows plan   (n <= 2^13): one launch of k_fft_rows, all passes in LDS;
+//   * pass plan   (n > 2^13): P launches of k_fft_pass, radix up to 2^13 each, the
+//                 Stockham pass formula of FFT_OpenCL_Contiguous.h:167-180 with its
+//                 A_q / N_q prefix products (computeParameters, OpenCLFFTAlgorithm.h:9-12).
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/estl_hip.h"
+#include "internal.h"
+#include "plan.h"
+
+namespace estl {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+const char *last_error() { return g_last_error.c_str(); }
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return set_error(ESTL_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                             __FILE__, __LINE__);                                                  \
+    } while (0)
+
+// ------------------------------------------------------------------ device state
+static thread_local int g_device = -1;
+
+int current_device() {
+    if (g_device < 0) {
+        int d = 0;
+        if (hipGetDevice(&d) != hipSuccess) d = 0;
+        g_device = d;
+    }
+    return g_device;
+}
+
+// Twiddle tables, built once per device and key, resident until process exit.
+struct TwiddleCache {
+    std::mutex mu;
+    std::map<std::tuple<int, int, long, long, long>, float2 *> tables;  // (device, kind, a, b, c)
+};
+static TwiddleCache &twcache() {
+    static TwiddleCache *c = new TwiddleCache;  // never destroyed: tables live to process exit
+    return *c;
+}
+
+// Table of BlockGeom<log2n>: W_{N_q}^{g s} at [(s-1) A_q + g] per pass q >= 1.
+int block_table(int device, int log2n, const float2 **out) {
+    auto &c = twcache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto key = std::make_tuple(device, 0, (long)log2n, 0L, 0L);
+    auto it = c.tables.find(key);
+    if (it != c.tables.end()) {
+        *out = it->second;
+        return ESTL_OK;
+    }
+    long count = rows_twiddle_count(log2n);
+    if (count < 0) return set_error(ESTL_ERR_UNSUPPORTED, "no block kernel for log2n=%d", log2n);
+    float2 *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(float2) * (count > 0 ? count : 1)));
+    long off = 0;
+    for (int q = 1; q < rows_pass_count(log2n); ++q) {
+        long A;
+        int B;
+        rows_pass_params(log2n, q, &A, &B);
+        long cnt = (long)(B - 1) * A;
+        HIP_TRY(launch_twiddle(d + off, cnt, 0, A, 0, A * B, nullptr));
+        off += cnt;
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    c.tables[key] = d;
+    *out = d;
+    return ESTL_OK;
+}
+
+// Linear table: entry e = exp(-2 pi i (e * mult mod nq) / nq), e < count.
+int linear_table(int device, long count, long mult, long nq, const float2 **out) {
+    auto &c = twcache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto key = std::make_tuple(device, 1, count, mult, nq);
+    auto it = c.tables.find(key);
+    if (it != c.tables.end()) {
+        *out = it->second;
+        return ESTL_OK;
+    }
+    float2 *d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(float2) * count));
+    HIP_TRY(launch_twiddle(d, count, 1, 1, mult, nq, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    c.tables[key] = d;
+    *out = d;
+    return ESTL_OK;
+}
+
+static int ilog2(long n) {
+    int l = 0;
+    while ((1L << l) < n) ++l;
+    return l;
+}
+
+// Split log2n into passes of at most 9 bits (512-point radices keep a 16-column tile
+// inside 64 KiB of LDS), as evenly as possible, larger radices first.
+std::vector<int> default_pass_split(int log2n) {
+    int P = (log2n + 8) / 9;
+    if (P < 2) P = 2;
+    std::vector<int> bits(P, log2n / P);
+    for (int i = 0; i < log2n % P; ++i) bits[i]++;
+    return bits;
+}
+
+int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned flags, const std::vector<int> *split) {
+    if (n < 1 || (n & (n - 1)))
+        return set_error(ESTL_ERR_INVALID, "n=%ld is not a power of two", n);
+    if (batch < 1) return set_error(ESTL_ERR_INVALID, "batch=%ld must be >= 1", batch);
+    if (dir != ESTL_FORWARD && dir != ESTL_BACKWARD)
+        return set_error(ESTL_ERR_INVALID, "direction=%d must be -1 or +1", dir);
+    if (precision != ESTL_PRECISION_FP32)
+        return set_error(ESTL_ERR_UNSUPPORTED, "precision %d not supported by this entry point", precision);
+    p->device = current_device();
+    p->n = n;
+    p->batch = batch;
+    p->dir = dir;
+    p->precision = precision;
+    p->flags = flags;
+    p->log2n = ilog2(n);
+    p->scale = (flags & ESTL_FLAG_NORMALIZE) ? 1.0f / (float)n : 1.0f;
+    if (n == 1) {
+        p->kind = Plan::COPY;
+        return ESTL_OK;
+    }
+    if (p->log2n <= kMaxRowsLog2 && !(split && !split->empty())) {
+        p->kind = Plan::ROWS;
+        return block_table(p->device, p->log2n, &p->tw_rows);
+    }
+    p->kind = Plan::PASSES;
+    std::vector<int> bits = (split && !split->empty()) ? *split : default_pass_split(p->log2n);
+    int total = 0;
+    for (int b : bits) {
+        if (b < 4 || b > kMaxPassLog2)
+            return set_error(ESTL_ERR_UNSUPPORTED, "pass radix 2^%d outside [2^4, 2^%d]", b, kMaxPassLog2);
+        total += b;
+    }
+    if (total != p->log2n) return set_error(ESTL_ERR_INVALID, "pass split does not sum to log2n=%d", p->log2n);
+    long A = 1;
+    for (size_t q = 0; q < bits.size(); ++q) {
+        PassStep st{};
+        st.log2b = bits[q];
+        const long B = 1L << bits[q];
+        st.args.n = n;
+        st.args.batch = batch;
+        st.args.A = A;
+        st.args.S = n / (A * B);
+        st.args.scale = (q + 1 == bits.size()) ? p->scale : 1.0f;
+        st.args.xcd_remap = 1;
+        const int T = pass_tile(bits[q]);
+        if ((n / B) % T) return set_error(ESTL_ERR_UNSUPPORTED, "tile %d does not divide %ld columns", T, n / B);
+        int rc = block_table(p->device, bits[q], &st.args.tw_local);
+        if (rc) return rc;
+        if (A > 1) {
+            const long nq = A * B;
+            const int lnq = ilog2(nq);
+            st.args.lobits = (lnq + 1) / 2;
+            rc = linear_table(p->device, 1L << st.args.lobits, 1, nq, &st.args.tw_lo);
+            if (rc) return rc;
+            rc = linear_table(p->device, nq >> st.args.lobits, 1L << st.args.lobits, nq, &st.args.tw_hi);
+            if (rc) return rc;
+        }
+        p->steps.push_back(st);
+        A *= B;
+    }
+    p->scratch_bytes = sizeof(float2) * (size_t)n * (size_t)batch;
+    return ESTL_OK;
+}
+
+static int ensure_scratch(Plan *p, int which) {
+    if (p->scratch[which]) return ESTL_OK;
+    HIP_TRY(hipMalloc(&p->scratch[which], p->scratch_bytes));
+    return ESTL_OK;
+}
+
+int execute_plan(Plan *p, const float2 *in, float2 *out, hipStream_t st) {
+    if (!in || !out) return set_error(ESTL_ERR_INVALID, "null buffer");
+    switch (p->kind) {
+        case Plan::COPY: {
+            if (p->scale == 1.0f && in != out)
+                HIP_TRY(hipMemcpyAsync(out, in, sizeof(float2) * p->batch, hipMemcpyDeviceToDevice, st));
+            return ESTL_OK;
+        }
+        case Plan::ROWS:
+            HIP_TRY(launch_rows(p->log2n, p->dir, in, out, p->batch, p->tw_rows, p->scale, st));
+            return ESTL_OK;
+        case Plan::PASSES: {
+            const int P = (int)p->steps.size();
+            int rc = ensure_scratch(p, 0);
+            if (rc) return rc;
+            // destinations backwards from out, alternating with scratch 0; the first pass
+            // must not write its own input, so an in-place odd-length chain uses scratch 1
+            std::vector<float2 *> dst(P);
+            for (int q = P - 1, k = 0; q >= 0; --q, ++k) dst[q] = (k & 1) ? p->scratch[0] : out;
+            if (dst[0] == in) {
+                rc = ensure_scratch(p, 1);
+                if (rc) return rc;
+                dst[0] = p->scratch[1];
+            }
+            const float2 *src = in;
+            for (int q = 0; q < P; ++q) {
+                HIP_TRY(launch_pass(p->steps[q].log2b, p->dir, src, dst[q], p->steps[q].args, st));
+                src = dst[q];
+            }
+            return ESTL_OK;
+        }
+    }
+    return set_error(ESTL_ERR_INVALID, "bad plan");
+}
+
+void destroy_plan(Plan *p) {
+    if (!p) return;
+    for (auto &s : p->scratch)
+        if (s) (void)hipFree(s);
+    if (p->h_in_dev) (void)hipFree(p->h_in_dev);
+    if (p->h_out_dev) (void)hipFree(p->h_out_dev);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+std::string describe_plan(const Plan *p) {
+    char buf[256];
+    std::string s = "{";
+    snprintf(buf, sizeof buf, "\"n\": %ld, \"batch\": %ld, \"direction\": %d, ", p->n, p->batch, p->dir);
+    s += buf;
+    if (p->kind == Plan::COPY) {
+        s += "\"kind\": \"copy\", \"passes\": []}";
+        return s;
+    }
+    if (p->kind == Plan::ROWS) {
+        s += "\"kind\": \"rows\", \"kernel\": \"k_fft_rows\", \"radices\": [";
+        for (int q = 0; q < rows_pass_count(p->log2n); ++q) {
+            long A;
+            int B;
+            rows_pass_params(p->log2n, q, &A, &B);
+            snprintf(buf, sizeof buf, "%s%d", q ? ", " : "", B);
+            s += buf;
+        }
+        s += "], \"launches\": 1}";
+        return s;
+    }
+    s += "\"kind\": \"passes\", \"kernel\": \"k_fft_pass\", \"passes\": [";
+    for (size_t q = 0; q < p->steps.size(); ++q) {
+        const auto &st = p->steps[q];
+        snprintf(buf, sizeof buf, "%s{\"radix\": %ld, \"A\": %ld, \"S\": %ld, \"tile\": %d}", q ? ", " : "",
+                 1L << st.log2b, st.args.A, st.args.S, pass_tile(st.log2b));
+        s += buf;
+    }
+    snprintf(buf, sizeof buf, "], \"launches\": %zu}", p->steps.size());
+    s += buf;
+    return s;
+}
+
+int plan_launches(const Plan *p) {
+    switch (p->kind) {
+        case Plan::COPY: return 0;
+        case Plan::ROWS: return 1;
+        default: return (int)p->steps.size();
+    }
+}
+
+}  // namespace estl
+
+using namespace estl;
+
+extern "C" {
+
+const char *estl_last_error(void) { return last_error(); }
+
+int estl_version(void) { return 100; }
+
+int estl_init(int device) {
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return set_error(ESTL_ERR_INVALID, "device %d out of range (%d devices)", device, count);
+    HIP_TRY(hipSetDevice(device));
+    g_device = device;
+    return ESTL_OK;
+}
+
+int estl_plan_create(estl_plan_t *plan, long n, long batch, int direction, int precision, unsigned flags) {
+    if (!plan) return set_error(ESTL_ERR_INVALID, "plan pointer is null");
+    *plan = nullptr;
+    Plan *p = new Plan();
+    int rc = build_plan(p, n, batch, direction, precision, flags, nullptr);
+    if (rc) {
+        destroy_plan(p);
+        return rc;
+    }
+    *plan = reinterpret_cast<estl_plan_t>(p);
+    return ESTL_OK;
+}
+
+int estl_plan_create_split(estl_plan_t *plan, long n, long batch, int direction, int precision, unsigned flags,
+                           int npasses, const int *log2radices) {
+    if (!plan) return set_error(ESTL_ERR_INVALID, "plan pointer is null");
+    *plan = nullptr;
+    if (npasses < 0 || npasses > 16 || (npasses > 0 && !log2radices))
+        return set_error(ESTL_ERR_INVALID, "bad pass split");
+    std::vector<int> split(log2radices, log2radices + npasses);
+    Plan *p = new Plan();
+    int rc = build_plan(p, n, batch, direction, precision, flags, &split);
+    if (rc) {
+        destroy_plan(p);
+        return rc;
+    }
+    *plan = reinterpret_cast<estl_plan_t>(p);
+    return ESTL_OK;
+}
+
+int estl_plan_destroy(estl_plan_t plan) {
+    destroy_plan(reinterpret_cast<Plan *>(plan));
+    return ESTL_OK;
+}
+
+int estl_execute_device(estl_plan_t plan, const void *d_in, void *d_out, void *stream) {
+    if (!plan) return set_error(ESTL_ERR_INVALID, "plan is null");
+    return execute_plan(reinterpret_cast<Plan *>(plan), static_cast<const float2 *>(d_in),
+                        static_cast<float2 *>(d_out), static_cast<hipStream_t>(stream));
+}
+
+int estl_execute_host(estl_plan_t plan, const void *h_in, void *h_out) {
+    if (!plan || !h_in || !h_out) return set_error(ESTL_ERR_INVALID, "null argument");
+    Plan *p = reinterpret_cast<Plan *>(plan);
+    const size_t bytes = sizeof(float2) * (size_t)p->n * (size_t)p->batch;
+    if (!p->stream) HIP_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    if (!p->h_in_dev) HIP_TRY(hipMalloc(&p->h_in_dev, bytes));
+    if (!p->h_out_dev) HIP_TRY(hipMalloc(&p->h_out_dev, bytes));
+    HIP_TRY(hipMemcpyAsync(p->h_in_dev, h_in, bytes, hipMemcpyHostToDevice, p->stream));
+    int rc = execute_plan(p, p->h_in_dev, p->h_out_dev, p->stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(h_out, p->h_out_dev, bytes, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return ESTL_OK;
+}
+
+long long estl_max_batch(long n, int precision, long long budget_bytes) {
+    if (n < 1 || (n & (n - 1))) return set_error(ESTL_ERR_INVALID, "n=%ld is not a power of two", n);
+    if (budget_bytes <= 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return set_error(ESTL_ERR_HIP, "hipMemGetInfo failed");
+        budget_bytes = (long long)fr;
+    }
+    const long long elem = (precision == ESTL_PRECISION_FP16) ? 4 : 8;
+    // input + output (+ one scratch row set for multi-pass sizes)
+    const long long per_row = elem * (long long)n * ((ilog2(n) <= kMaxRowsLog2) ? 2 : 3);
+    return budget_bytes / per_row;
+}
+
+long long estl_plan_workspace_bytes(estl_plan_t plan) {
+    if (!plan) return set_error(ESTL_ERR_INVALID, "plan is null");
+    return (long long)reinterpret_cast<Plan *>(plan)->scratch_bytes;
+}
+
+int estl_plan_describe(estl_plan_t plan, char *buf, int len) {
+    if (!plan) return set_error(ESTL_ERR_INVALID, "plan is null");
+    std::string s = describe_plan(reinterpret_cast<Plan *>(plan));
+    if (buf && len > 0) {
+        strncpy(buf, s.c_str(), (size_t)len - 1);
+        buf[len - 1] = 0;
+    }
+    return (int)s.size();
+}
+
+int estl_plan_launches(estl_plan_t plan) {
+    if (!plan) return set_error(ESTL_ERR_INVALID, "plan is null");
+    return plan_launches(reinterpret_cast<Plan *>(plan));
+}
+
+int estl_fill_uniform(void *d_dst, long count, uint64_t seed, uint64_t offset, void *stream) {
+    if (!d_dst && count > 0) return set_error(ESTL_ERR_INVALID, "null buffer");
+    HIP_TRY(launch_fill_uniform(static_cast<float *>(d_dst), count, seed, offset, static_cast<hipStream_t>(stream)));
+    return ESTL_OK;
+}
+
+}  // extern "C"

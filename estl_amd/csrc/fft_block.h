@@ -1,0 +1,100 @@
+// fft_block.h -- workgroup-cooperative Stockham FFT held in registers, exchanged
+// through LDS.
+//
+// A sequence of length N = 2^LOG2N is spread over TPT = N / E threads, E = 16 values
+// per thread: thread tl holds element tl + TPT*i in v[i].  The plan is the reference's
+// global radix list (FFT_OpenCL_Contiguous.h:55-60: radix-16 passes, the 2^(log2N mod 4)
+// remainder last) and every pass follows its Stockham recurrence
+// (FFT_OpenCL_Contiguous.h:167-180, docs/ImplementationStrategy.pdf eq. 7/12):
+//
+//   d_q[(N/N_q)(g + A_q h) + z] = sum_s d_{q-1}[(N/N_q)(B_q g + s) + z] W_{N_q}^{g s} W_{B_q}^{h s}
+//
+// but where the reference runs one kernel launch (one HBM round trip) per pass, here a
+// pass is a register DFT and the hand-over between passes is an LDS exchange.  After
+// every pass thread tl holds d_q[tl + TPT*i] in v[i] again, so the first pass needs no
+// exchange (its inputs are exactly what a coalesced load gives) and the last pass
+// leaves the result in the order a coalesced store wants.
+#pragma once
+
+#include "fft_device.h"
+
+namespace estl {
+
+template <int LOG2N>
+struct BlockGeom {
+    static constexpr int N = 1 << LOG2N;
+    static constexpr int E = (LOG2N >= 4) ? 16 : N;  // complex values per thread
+    static constexpr int TPT = N / E;                // threads per sequence
+    static constexpr int NFULL = LOG2N / 4;          // radix-16 passes
+    static constexpr int REM = LOG2N % 4;
+    static constexpr int P = NFULL + (REM ? 1 : 0);  // passes
+    static constexpr int radix(int q) { return q < NFULL ? 16 : (1 << REM); }
+    static constexpr long A(int q) {
+        long a = 1;
+        for (int p = 0; p < q; ++p) a *= radix(p);
+        return a;
+    }
+    // offset of pass q's twiddles in the table; pass q stores W_{N_q}^{g s} at
+    // [(s - 1) * A_q + g] for s in [1, B_q), g in [0, A_q) (pass 0 needs none)
+    static constexpr long twoff(int q) {
+        long o = 0;
+        for (int p = 1; p < q; ++p) o += (radix(p) - 1) * A(p);
+        return o;
+    }
+    static constexpr long twsize = twoff(P);
+    // LDS: one float2 of padding after every 16 values breaks the power-of-two
+    // strides of the exchange reads into distinct banks
+    static constexpr int PADSHIFT = 4;
+    static constexpr int lds_per_seq = (TPT > 1) ? (N + (N >> PADSHIFT)) : 0;
+    static __device__ __forceinline__ int pad(int i) { return i + (i >> PADSHIFT); }
+};
+
+// On entry v[i] = x[tl + TPT*i]; on exit v[i] = X[tl + TPT*i] (X = DFT of x, sign DIR).
+// `lds` is this sequence's LDS slice (lds_per_seq float2); `tw` the size-N table.
+// All threads of the workgroup must call it together (it holds workgroup barriers).
+// If `lds_busy` is set, the slice is still being read on entry and a barrier is placed
+// before the first LDS write.
+template <int LOG2N, int DIR>
+__device__ __forceinline__ void block_fft(float2 (&v)[BlockGeom<LOG2N>::E], float2 *lds, int tl,
+                                          const float2 *__restrict__ tw, bool lds_busy = false) {
+    using G = BlockGeom<LOG2N>;
+    constexpr int N = G::N, E = G::E, TPT = G::TPT;
+    StaticFor<0, G::P>::run([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int B = G::radix(q);
+        constexpr int NB = E / B;  // butterflies per thread in this pass
+        constexpr int Aq = int(G::A(q));
+        constexpr int S = N / (Aq * B);
+        if constexpr (q == 0) {
+            StaticFor<0, NB>::run([&](auto bc) {
+                constexpr int b = decltype(bc)::value;
+                float2 u[B];
+                StaticFor<0, B>::run([&](auto sc) { u[decltype(sc)::value] = v[b + NB * decltype(sc)::value]; });
+                Dft<B, DIR>::run(u);
+                StaticFor<0, B>::run([&](auto hc) { v[b + NB * decltype(hc)::value] = u[decltype(hc)::value]; });
+            });
+        } else {
+            if (q > 1 || lds_busy) __syncthreads();
+#pragma unroll
+            for (int i = 0; i < E; ++i) lds[G::pad(tl + TPT * i)] = v[i];
+            __syncthreads();
+            StaticFor<0, NB>::run([&](auto bc) {
+                constexpr int b = decltype(bc)::value;
+                const int j = tl + TPT * b;
+                const int g = j / S, z = j % S;   // S is a power of two
+                const int base = S * B * g + z;
+                float2 u[B];
+#pragma unroll
+                for (int s = 0; s < B; ++s) u[s] = lds[G::pad(base + S * s)];
+                const float2 *twq = tw + G::twoff(q) + g;
+#pragma unroll
+                for (int s = 1; s < B; ++s) u[s] = ctw<DIR>(u[s], twq[(s - 1) * Aq]);
+                Dft<B, DIR>::run(u);
+#pragma unroll
+                for (int h = 0; h < B; ++h) v[b + NB * h] = u[h];
+            });
+        }
+    });
+}
+
+}  // namespace estl

@@ -1,0 +1,47 @@
+// plan.h -- host-side plan object behind estl_plan_t.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace estl {
+
+struct PassStep {
+    int log2b;
+    PassArgs args;
+};
+
+struct Plan {
+    enum Kind { COPY, ROWS, PASSES };
+    int device = 0;
+    long n = 0, batch = 0;
+    int dir = -1, precision = 0, log2n = 0;
+    unsigned flags = 0;
+    float scale = 1.0f;
+    Kind kind = COPY;
+    const float2 *tw_rows = nullptr;
+    std::vector<PassStep> steps;
+    size_t scratch_bytes = 0;
+    float2 *scratch[2] = {nullptr, nullptr};
+    // resident buffers and stream of the synchronous host entry point
+    float2 *h_in_dev = nullptr, *h_out_dev = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+int set_error(int code, const char *fmt, ...);
+int current_device();
+int block_table(int device, int log2n, const float2 **out);
+int linear_table(int device, long count, long mult, long nq, const float2 **out);
+std::vector<int> default_pass_split(int log2n);
+int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned flags, const std::vector<int> *split);
+int execute_plan(Plan *p, const float2 *in, float2 *out, hipStream_t st);
+void destroy_plan(Plan *p);
+std::string describe_plan(const Plan *p);
+int plan_launches(const Plan *p);
+
+}  // namespace estl

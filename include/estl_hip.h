@@ -1,0 +1,119 @@
+/*
+ * estl_hip.h -- C ABI of the gfx950 FFT hot path (libestl_hip.so).
+ *
+ * Plain C: pointers, sizes and status codes; no C++ or torch types cross this
+ * boundary and no exception escapes it.  Every call returns ESTL_OK (0) or a negative
+ * status; estl_last_error() gives the message of the calling thread's last failure.
+ *
+ * Conventions (those of the reference's FFTW3 path, FFT_FFTW3.h:26-28):
+ *   - data is interleaved complex64 (re, im float pairs), rows of length n contiguous,
+ *     row r of a batch at offset r * n (idist = odist = n);
+ *   - direction ESTL_FORWARD = -1 computes X[k] = sum_t x[t] exp(-2 pi i t k / n),
+ *     ESTL_BACKWARD = +1 the same with exp(+...), both unnormalised unless
+ *     ESTL_FLAG_NORMALIZE asks for the 1/n factor;
+ *   - n is a power of two (the reference's plan assumes it: ceilog in
+ *     FFT_OpenCL_Contiguous.h:56, SimpleMath.h:7-11); other sizes are rejected.
+ *
+ * Deliberate divergences from FFT_OpenCL_Contiguous, the GPU backend being replaced:
+ *   - the backward transform is computed for every batch row (the reference conjugates
+ *     only the first row, FFT_OpenCL_Contiguous.h:100-102,134-136);
+ *   - the device-memory query is 64-bit (uOpenCL.h:323-337 truncates it to int).
+ */
+#ifndef ESTL_HIP_H
+#define ESTL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+#define ESTL_OK 0
+#define ESTL_ERR_INVALID (-1)     /* bad argument: size not a power of two, null pointer, ... */
+#define ESTL_ERR_HIP (-2)         /* HIP runtime error (launch, copy, allocation)           */
+#define ESTL_ERR_NOMEM (-3)       /* device memory exhausted or over the admission budget   */
+#define ESTL_ERR_UNSUPPORTED (-4) /* size/precision combination without a kernel            */
+#define ESTL_ERR_BUSY (-5)        /* server queue full (back-pressure)                      */
+
+/* ---------------------------------------------------------------- enums */
+#define ESTL_FORWARD (-1)
+#define ESTL_BACKWARD (+1)
+
+#define ESTL_PRECISION_FP32 0      /* complex64 storage, FP32 arithmetic                       */
+#define ESTL_PRECISION_FP16 1      /* half2 storage + per-row FP32 scale, FP32 arithmetic      */
+
+#define ESTL_FLAG_NORMALIZE 0x1u   /* multiply the result by 1/n                               */
+
+typedef struct estl_plan_s *estl_plan_t;
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *estl_last_error(void);
+
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int estl_version(void);
+
+/* Select the HIP device used by plans created afterwards on this thread and build its
+ * resident state.  Replaces the context/queue set-up of OpenCLAlgorithm's constructor
+ * (uOpenCL.h:284-321), which the reference repeats for every FFT object. */
+int estl_init(int device);
+
+/* Create a plan for `batch` rows of length n.  Replaces the constructor
+ * FFT_OpenCL_Contiguous(int n, bool forward, int batch) (FFT_OpenCL_Contiguous.h:46-71,
+ * plan at :55-61) and the factory call FFTFactorySpecific::newFFT (FFT.h:51).
+ * direction: ESTL_FORWARD / ESTL_BACKWARD.  precision: ESTL_PRECISION_*.
+ * flags: ESTL_FLAG_* bits.  Twiddle tables are built on the device (FP64, rounded to
+ * FP32) and shared by all plans of the same size on that device. */
+int estl_plan_create(estl_plan_t *plan, long n, long batch, int direction, int precision, unsigned flags);
+
+/* Same as estl_plan_create with an explicit pass split: npasses global passes of radix
+ * 2^log2radices[q] (each in [4, 13], summing to log2 n).  npasses = 0 picks the
+ * default plan.  The reference fixes this split to radix-16 passes with the remainder
+ * last (BGs, FFT_OpenCL_Contiguous.h:55-60); here it is the plan space the autotuner
+ * searches. */
+int estl_plan_create_split(estl_plan_t *plan, long n, long batch, int direction, int precision, unsigned flags,
+                           int npasses, const int *log2radices);
+
+/* Destroy a plan (FFT_OpenCL_Contiguous::~FFT_OpenCL_Contiguous, FFT_OpenCL_Contiguous.h:186-191). */
+int estl_plan_destroy(estl_plan_t plan);
+
+/* Asynchronous execute on device buffers: d_in and d_out hold batch*n interleaved
+ * complex64 values (batch*n half2 values plus batch floats of scale in FP16 mode, see
+ * estl_plan_fp16_scales).  stream is a hipStream_t (NULL = the default stream).
+ * d_in == d_out is allowed.  Replaces the pass-launch loop of
+ * FFT_OpenCL_Contiguous::execute (FFT_OpenCL_Contiguous.h:108-124). */
+int estl_execute_device(estl_plan_t plan, const void *d_in, void *d_out, void *stream);
+
+/* Synchronous execute on host buffers with the reference's semantics: copy in, run,
+ * copy out, wait (FFT_OpenCL_Contiguous::execute, FFT_OpenCL_Contiguous.h:97-142, and
+ * FFT_FFTW3::execute, FFT_FFTW3.h:31).  The plan keeps its device buffers resident. */
+int estl_execute_host(estl_plan_t plan, const void *h_in, void *h_out);
+
+/* Largest batch of length-n rows whose working set fits `budget_bytes` of device memory
+ * (budget_bytes <= 0: the device's free memory).  64-bit replacement of
+ * FFT_OpenCL_Contiguous::getMaxBatchSize / getRequiredMemory (FFT_OpenCL_Contiguous.h:43-44). */
+long long estl_max_batch(long n, int precision, long long budget_bytes);
+
+/* Bytes of device memory a plan's execution needs beyond its inputs and outputs. */
+long long estl_plan_workspace_bytes(estl_plan_t plan);
+
+/* JSON description of the plan (kind, passes, radices, kernels) into buf (NUL-terminated).
+ * Returns the full length; the reference exposes the per-pass radix as getKernelInfo
+ * (FFT_OpenCL_Contiguous.h:95). */
+int estl_plan_describe(estl_plan_t plan, char *buf, int len);
+
+/* Number of kernel launches one execute makes (the reference: one per pass,
+ * FFT_OpenCL_Contiguous.h:108). */
+int estl_plan_launches(estl_plan_t plan);
+
+/* Fill count floats at d_dst with the seeded uniform[-1,1) stream:
+ * value i = ((u >> 40) * 2^-23) - 1, u = splitmix64(splitmix64(seed) + offset + i).
+ * No reference counterpart (the reference seeds rand() with time(NULL), tests/Test.cpp:189). */
+int estl_fill_uniform(void *d_dst, long count, uint64_t seed, uint64_t offset, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ESTL_HIP_H */
