@@ -77,6 +77,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--percall-rows", type=int, default=BATCH)
+    ap.add_argument("--counters", action="store_true",
+                    help="counter-collection mode for rocprofv3 --pmc: 3 forward launches of the "
+                         "batched plan and 3 plain 512 MiB device copies (the calibration kernel), nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -102,6 +105,17 @@ def main() -> int:
     fwd = estl_amd.Plan(N, BATCH, estl_amd.FORWARD)
     bwd = estl_amd.Plan(N, BATCH, estl_amd.BACKWARD)
     plan_desc = fwd.describe()
+
+    if args.counters:
+        with torch.cuda.stream(stream):
+            for _ in range(3):
+                fwd.execute(x, y, stream)
+            for _ in range(3):
+                z.copy_(x)
+        stream.synchronize()
+        print(json.dumps({"counters_mode": True, "launch_bytes": 16 * N * BATCH,
+                          "copy_bytes_read": 8 * N * BATCH, "copy_bytes_written": 8 * N * BATCH}), flush=True)
+        return 0
 
     def step():
         fwd.execute(x, y, stream)

@@ -42,19 +42,37 @@ struct BlockGeom {
         return o;
     }
     static constexpr long twsize = twoff(P);
-    // LDS: one float2 of padding after every 16 values breaks the power-of-two
-    // strides of the exchange reads into distinct banks
-    static constexpr int PADSHIFT = 4;
-    static constexpr int lds_per_seq = (TPT > 1) ? (N + (N >> PADSHIFT)) : 0;
-    static __device__ __forceinline__ int pad(int i) { return i + (i >> PADSHIFT); }
+    static constexpr bool uses_lds = TPT > 1;
+};
+
+// LDS layouts of a sequence's exchange buffer.
+//  kLdsPad: one float2 of padding after every 16 values; addresses stay base + constant,
+//           so the exchange reads use immediate offsets (fewest VGPRs);
+//  kLdsXor: no padding, value i sits in row i/16 at column (i mod 16) XOR (row mod 16).
+//           Every exchange of these passes touches, per 16 lanes, either 16 consecutive
+//           values of one row or one value from each of 16 rows at a fixed column; the
+//           XOR spreads both over 16 distinct columns (all 32 four-byte banks for 8-byte
+//           values) and a 4096-value sequence needs exactly 32 KiB.
+enum { kLdsPad = 0, kLdsXor = 1 };
+template <int LAYOUT>
+struct Lds;
+template <>
+struct Lds<kLdsPad> {
+    static constexpr int size(int n) { return n + (n >> 4); }
+    static __device__ __forceinline__ int idx(int i) { return i + (i >> 4); }
+};
+template <>
+struct Lds<kLdsXor> {
+    static constexpr int size(int n) { return n; }
+    static __device__ __forceinline__ int idx(int i) { return i ^ ((i >> 4) & 15); }
 };
 
 // On entry v[i] = x[tl + TPT*i]; on exit v[i] = X[tl + TPT*i] (X = DFT of x, sign DIR).
-// `lds` is this sequence's LDS slice (lds_per_seq float2); `tw` the size-N table.
+// `lds` is this sequence's LDS slice (Lds<LAYOUT>::size(N) float2); `tw` the size-N table.
 // All threads of the workgroup must call it together (it holds workgroup barriers).
 // If `lds_busy` is set, the slice is still being read on entry and a barrier is placed
 // before the first LDS write.
-template <int LOG2N, int DIR>
+template <int LOG2N, int DIR, int LAYOUT = kLdsPad>
 __device__ __forceinline__ void block_fft(float2 (&v)[BlockGeom<LOG2N>::E], float2 *lds, int tl,
                                           const float2 *__restrict__ tw, bool lds_busy = false) {
     using G = BlockGeom<LOG2N>;
@@ -76,7 +94,7 @@ __device__ __forceinline__ void block_fft(float2 (&v)[BlockGeom<LOG2N>::E], floa
         } else {
             if (q > 1 || lds_busy) __syncthreads();
 #pragma unroll
-            for (int i = 0; i < E; ++i) lds[G::pad(tl + TPT * i)] = v[i];
+            for (int i = 0; i < E; ++i) lds[Lds<LAYOUT>::idx(tl + TPT * i)] = v[i];
             __syncthreads();
             StaticFor<0, NB>::run([&](auto bc) {
                 constexpr int b = decltype(bc)::value;
@@ -85,7 +103,7 @@ __device__ __forceinline__ void block_fft(float2 (&v)[BlockGeom<LOG2N>::E], floa
                 const int base = S * B * g + z;
                 float2 u[B];
 #pragma unroll
-                for (int s = 0; s < B; ++s) u[s] = lds[G::pad(base + S * s)];
+                for (int s = 0; s < B; ++s) u[s] = lds[Lds<LAYOUT>::idx(base + S * s)];
                 const float2 *twq = tw + G::twoff(q) + g;
 #pragma unroll
                 for (int s = 1; s < B; ++s) u[s] = ctw<DIR>(u[s], twq[(s - 1) * Aq]);

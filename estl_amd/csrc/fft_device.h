@@ -40,6 +40,17 @@ __device__ __forceinline__ float2 ctw(float2 a, float2 w) {
     else return cmulc(a, w);
 }
 
+// Streaming (nontemporal) global accesses for data touched exactly once.
+typedef float v2f_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_stream(const float2 *p) {
+    v2f_t v = __builtin_nontemporal_load(reinterpret_cast<const v2f_t *>(p));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(float2 *p, float2 x) {
+    v2f_t v = {x.x, x.y};
+    __builtin_nontemporal_store(v, reinterpret_cast<v2f_t *>(p));
+}
+
 // ---------------------------------------------------------------- compile-time trig
 namespace ct {
 constexpr double kPi = 3.141592653589793238462643383279502884;

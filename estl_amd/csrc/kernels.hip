@@ -47,37 +47,74 @@ __global__ void k_fill_uniform(float *__restrict__ dst, long count, uint64_t key
 }
 
 // ------------------------------------------------------------------ batched rows
-// TPW sequences per workgroup, TPT threads each; blockIdx.x covers TPW rows.
-template <int LOG2N, int DIR, int TPW>
+// A group is TPW rows, one per TPT threads.  The grid is persistent (sized to the
+// resident workgroup count by the launcher): each workgroup walks groups
+// blockIdx.x, blockIdx.x + gridDim.x, ... and issues the loads of its next group
+// before running the current one through LDS, so HBM reads stay in flight during the
+// register/LDS passes.  Input and output are touched once, so both use nontemporal
+// (streaming) accesses.  Rows past the batch end (a partial last group) are computed
+// on whatever the clamped load returned and never stored.
+template <int LOG2N, int DIR, int TPW, bool SCALED>
 __global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT * TPW)
 k_fft_rows(const float2 *__restrict__ in, float2 *__restrict__ out, long batch,
            const float2 *__restrict__ tw, float scale) {
     using G = BlockGeom<LOG2N>;
     constexpr int N = G::N, E = G::E, TPT = G::TPT;
-    constexpr int LDS = (G::lds_per_seq > 0) ? TPW * G::lds_per_seq : 1;
+    constexpr int SLICE = G::uses_lds ? Lds<kLdsPad>::size(N) : 0;
+    constexpr int LDS = G::uses_lds ? TPW * SLICE : 1;
     __shared__ float2 lds[LDS];
-    const int t = threadIdx.x;
-    const int w = t / TPT, tl = t % TPT;
-    const long row = (long)blockIdx.x * TPW + w;
-    const bool active = row < batch;
-    const float2 *src = in + row * N;
-    float2 v[E];
-    if (active) {
+    float2 v[E], vn[E];
+    bool busy = false;
+    if constexpr (TPW == 1) {
+        // one row per workgroup and group
+        const int tl = threadIdx.x;
+        long row = blockIdx.x;
+        if (row < batch) {
 #pragma unroll
-        for (int i = 0; i < E; ++i) v[i] = src[tl + TPT * i];
+            for (int i = 0; i < E; ++i) v[i] = ld_stream(in + row * N + tl + TPT * i);
+        }
+        for (; row < batch; row += gridDim.x) {
+            const long nr = row + gridDim.x;
+            if (nr < batch) {
+#pragma unroll
+                for (int i = 0; i < E; ++i) vn[i] = ld_stream(in + nr * N + tl + TPT * i);
+            }
+            block_fft<LOG2N, DIR, kLdsPad>(v, lds, tl, tw, busy);
+            busy = true;
+            float2 *dst = out + row * N + tl;
+#pragma unroll
+            for (int i = 0; i < E; ++i) st_stream(dst + TPT * i, SCALED ? cscale(v[i], scale) : v[i]);
+#pragma unroll
+            for (int i = 0; i < E; ++i) v[i] = vn[i];
+        }
     } else {
+        const int t = threadIdx.x;
+        const int w = t / TPT, tl = t % TPT;
+        const long groups = (batch + TPW - 1) / TPW;
+        const long last = batch - 1;
+        long grp = blockIdx.x;
+        if (grp < groups) {
+            const float2 *src = in + min(grp * TPW + w, last) * N + tl;
 #pragma unroll
-        for (int i = 0; i < E; ++i) v[i] = make_float2(0.f, 0.f);
-    }
-    block_fft<LOG2N, DIR>(v, lds + w * G::lds_per_seq, tl, tw);
-    if (active) {
-        float2 *dst = out + row * N;
-        if (scale != 1.0f) {
+            for (int i = 0; i < E; ++i) v[i] = ld_stream(src + TPT * i);
+        }
+        for (; grp < groups; grp += gridDim.x) {  // uniform across the workgroup
+            const long row = grp * TPW + w;
+            const long next = grp + gridDim.x;
+            if (next < groups) {
+                const float2 *src = in + min(next * TPW + w, last) * N + tl;
 #pragma unroll
-            for (int i = 0; i < E; ++i) dst[tl + TPT * i] = cscale(v[i], scale);
-        } else {
+                for (int i = 0; i < E; ++i) vn[i] = ld_stream(src + TPT * i);
+            }
+            block_fft<LOG2N, DIR, kLdsPad>(v, lds + w * SLICE, tl, tw, busy);
+            busy = true;
+            if (row < batch) {
+                float2 *dst = out + row * N + tl;
 #pragma unroll
-            for (int i = 0; i < E; ++i) dst[tl + TPT * i] = v[i];
+                for (int i = 0; i < E; ++i) st_stream(dst + TPT * i, SCALED ? cscale(v[i], scale) : v[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < E; ++i) v[i] = vn[i];
         }
     }
 }
@@ -96,7 +133,7 @@ __global__ void __launch_bounds__(BlockGeom<LOG2B>::TPT * T)
 k_fft_pass(const float2 *__restrict__ in, float2 *__restrict__ out, PassArgs a) {
     using G = BlockGeom<LOG2B>;
     constexpr int B = G::N, E = G::E, TPT = G::TPT;
-    constexpr int LDSP = B + (B >> G::PADSHIFT);  // block_fft's padded slice (also used to stage loads)
+    constexpr int LDSP = Lds<kLdsPad>::size(B);  // one column's block_fft slice (also stages loads)
     __shared__ float2 lds[T * LDSP];
     const int t = threadIdx.x;
     const int c = t % T, tl = t / T;
@@ -132,11 +169,11 @@ k_fft_pass(const float2 *__restrict__ in, float2 *__restrict__ out, PassArgs a) 
             const int o = t + NT * k;
             const int gl = o / (int(S) * B), rem = o - gl * (int(S) * B);
             const int s = rem / int(S), zz = rem - s * int(S);
-            lds[(gl * int(S) + zz) * LDSP + G::pad(s)] = p[o];
+            lds[(gl * int(S) + zz) * LDSP + Lds<kLdsPad>::idx(s)] = p[o];
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < E; ++i) v[i] = lds[c * LDSP + G::pad(tl + TPT * i)];
+        for (int i = 0; i < E; ++i) v[i] = lds[c * LDSP + Lds<kLdsPad>::idx(tl + TPT * i)];
         lds_busy = true;
     }
     if (a.A > 1) {
@@ -148,7 +185,7 @@ k_fft_pass(const float2 *__restrict__ in, float2 *__restrict__ out, PassArgs a) 
             v[i] = ctw<DIR>(v[i], w);
         }
     }
-    block_fft<LOG2B, DIR>(v, lds + c * LDSP, tl, a.tw_local, lds_busy);
+    block_fft<LOG2B, DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, lds_busy);
     float2 *dst = out + r * a.n + j;
     const long stride = cols_per_row;
     if (a.scale != 1.0f) {
@@ -161,14 +198,36 @@ k_fft_pass(const float2 *__restrict__ in, float2 *__restrict__ out, PassArgs a) 
 }
 
 // ------------------------------------------------------------------ launchers
+// Workgroups of `kernel` resident on the whole device at once (occupancy x CUs),
+// cached per kernel and device.
+template <class K>
+static long resident_blocks(K kernel, int block_threads) {
+    static long cache[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cache[dev] == 0) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block_threads, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+        cache[dev] = (long)per_cu * cus;
+    }
+    return cache[dev];
+}
+
+// persistent grid: this many resident rounds of workgroups
+constexpr long kRowsGridRounds = 4;
+
 template <int LOG2N, int DIR>
 static hipError_t launch_rows_t(const float2 *in, float2 *out, long batch, const float2 *tw, float scale,
                                 hipStream_t st) {
     using G = BlockGeom<LOG2N>;
     constexpr int TPW = (G::TPT >= 256) ? 1 : 256 / G::TPT;
-    const long blocks = (batch + TPW - 1) / TPW;
-    hipLaunchKernelGGL((k_fft_rows<LOG2N, DIR, TPW>), dim3((unsigned)blocks), dim3(G::TPT * TPW), 0, st, in, out,
-                       batch, tw, scale);
+    const long groups = (batch + TPW - 1) / TPW;
+    auto kernel = (scale != 1.0f) ? k_fft_rows<LOG2N, DIR, TPW, true> : k_fft_rows<LOG2N, DIR, TPW, false>;
+    long blocks = resident_blocks(k_fft_rows<LOG2N, DIR, TPW, false>, G::TPT * TPW) * kRowsGridRounds;
+    if (blocks > groups) blocks = groups;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(G::TPT * TPW), 0, st, in, out, batch, tw, scale);
     return hipGetLastError();
 }
 

@@ -172,8 +172,8 @@ def test_batched_4096_full_size(gpu):
     ey = (y.abs().double() ** 2).sum(dim=1)
     assert torch.max(torch.abs(ey / (n * ex) - 1)).item() < 1e-5
     z = run(torch, b, y)
-    num = torch.linalg.vector_norm((z / n - x).double()).item()
-    den = torch.linalg.vector_norm(x.double()).item()
+    num = torch.linalg.vector_norm((z / n - x).to(torch.complex128)).item()
+    den = torch.linalg.vector_norm(x.to(torch.complex128)).item()
     assert num / den <= 2 * tol(n)
 
 
@@ -209,8 +209,8 @@ def test_large_single_2_26(gpu):
     ey = (y.abs().double() ** 2).sum().item()
     assert abs(ey / (n * ex) - 1) < 1e-5
     z = run(torch, b, y)
-    num = torch.linalg.vector_norm((z / n - x).double()).item()
-    den = torch.linalg.vector_norm(x.double()).item()
+    num = torch.linalg.vector_norm((z / n - x).to(torch.complex128)).item()
+    den = torch.linalg.vector_norm(x.to(torch.complex128)).item()
     assert num / den <= 2 * tol(n)
 
 
