@@ -70,12 +70,73 @@ def cpu_baseline(rows: int, reps: int):
                       f"{rows} rows x N={N}, forward + backward, median of {reps} runs, 1 thread"}
 
 
+def bench_uneven(torch, estl_amd, stream, reps: int = 10):
+    """8192 transforms, N = 2^U{8..18} (seed 3), 10 % unit tones: one graph-captured
+    work-queue launch vs one plan launch per transform (same stream, device-resident)."""
+    import numpy as np
+    import estl_amd.workloads as wl
+    b = wl.uneven_batch()
+    x = torch.empty(b.total, dtype=torch.complex64, device="cuda:0")
+    y = torch.empty_like(x)
+    with torch.cuda.stream(stream):
+        wl.fill_uneven(b, x, torch)
+    stream.synchronize()
+    ub = estl_amd.UnevenBatch(b.sizes.tolist())
+    for _ in range(2):
+        ub.execute(x, y, stream, graph=True)
+    stream.synchronize()
+    times = []
+    for _ in range(reps):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        ub.execute(x, y, stream, graph=True)
+        e.record(stream)
+        times.append((s, e))
+    stream.synchronize()
+    ms = sorted(s.elapsed_time(e) for s, e in times)
+    t_batched = ms[len(ms) // 2] * 1e-3
+    # tones: exact spectrum N at bin f
+    worst = 0.0
+    for i in np.nonzero(b.tone)[0][:200]:
+        n, off, f = int(b.sizes[i]), int(b.offsets[i]), int(b.freq[i])
+        yi = y[off:off + n]
+        peak = yi[f].item()
+        err2 = (yi.abs().double() ** 2).sum().item() - abs(peak) ** 2 + abs(peak - n) ** 2
+        worst = max(worst, math.sqrt(max(err2, 0.0)) / n)
+    # per-call form
+    plans = {}
+    for n in sorted(set(b.sizes.tolist())):
+        plans[n] = estl_amd.Plan(n, 1, estl_amd.FORWARD)
+    xb, yb = x.data_ptr(), y.data_ptr()
+    launches = sum(plans[n].launches() for n in b.sizes.tolist())
+
+    def percall():
+        for n, off in zip(b.sizes.tolist(), b.offsets.tolist()):
+            plans[n].execute(xb + 8 * off, yb + 8 * off, stream)
+
+    percall()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    percall()
+    stream.synchronize()
+    t_percall = time.perf_counter() - t0
+    bytes_ = 16.0 * b.total
+    return {"workload": "8192 C2C fp32 transforms, N = 2^U{8..18} (seed 3), 10% unit tones, packed",
+            "elements": b.total, "batched_ms_p50": round(t_batched * 1e3, 4),
+            "batched_ms_min": round(ms[0], 4),
+            "batched_gbs": round(bytes_ / t_batched / 1e9, 1), "launches_batched": 1,
+            "per_call_ms": round(t_percall * 1e3, 2), "per_call_gbs": round(bytes_ / t_percall / 1e9, 2),
+            "launches_per_call": launches, "batched_over_per_call": round(t_percall / t_batched, 1),
+            "tone_rel_l2_max": worst, "work_items": ub.info["items"], "workgroups": ub.info["workgroups"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-size batch leg")
     ap.add_argument("--percall-rows", type=int, default=BATCH)
     ap.add_argument("--counters", action="store_true",
                     help="counter-collection mode for rocprofv3 --pmc: 3 forward launches of the "
@@ -201,6 +262,8 @@ def main() -> int:
     lat_us = [1e3 * s.elapsed_time(e) for s, e in lat]
     err_c1 = oracle.rel_l2(o1.cpu().numpy()[0], oracle.fft64(a1.cpu().numpy()[0]))
 
+    uneven = None if args.no_uneven else bench_uneven(torch, estl_amd, stream)
+
     cpu = None if args.no_cpu else cpu_baseline(BATCH, 3)
 
     out = {
@@ -233,6 +296,7 @@ def main() -> int:
                        "n65536_batch1_p99": round(percentile(lat_us, 99), 2)},
         "rel_l2_vs_fp64": {"forward_sampled_rows": err_fwd, "round_trip": rt, "n65536": err_c1,
                            "bound": 1e-6 * math.log2(N)},
+        "uneven_batch": uneven,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -23,6 +23,7 @@ BACKWARD = +1
 FP32 = 0
 FP16 = 1
 FLAG_NORMALIZE = 0x1
+FLAG_AUTOTUNE = 0x2
 
 OK = 0
 ERR_INVALID = -1
@@ -61,7 +62,8 @@ def lib():
     _sig(L, "estl_version", ci, [])
     _sig(L, "estl_init", ci, [ci])
     _sig(L, "estl_plan_create", ci, [ctypes.POINTER(vp), cl, cl, ci, ci, cu])
-    _sig(L, "estl_plan_create_split", ci, [ctypes.POINTER(vp), cl, cl, ci, ci, cu, ci, ctypes.POINTER(ci)])
+    _sig(L, "estl_plan_create_split", ci, [ctypes.POINTER(vp), cl, cl, ci, ci, cu, ci, ctypes.POINTER(ci),
+                                              ctypes.POINTER(ci)])
     _sig(L, "estl_plan_destroy", ci, [vp])
     _sig(L, "estl_execute_device", ci, [vp, vp, vp, vp])
     _sig(L, "estl_execute_host", ci, [vp, vp, vp])
@@ -70,6 +72,13 @@ def lib():
     _sig(L, "estl_plan_describe", ci, [vp, ctypes.c_char_p, ci])
     _sig(L, "estl_plan_launches", ci, [vp])
     _sig(L, "estl_fill_uniform", ci, [vp, cl, u64, u64, vp])
+    _sig(L, "estl_tune", ci, [cl, cl, ci, ctypes.c_char_p, ci])
+    lp = ctypes.POINTER(ctypes.c_long)
+    _sig(L, "estl_ubatch_create", ci, [ctypes.POINTER(vp), ci, lp, lp, ci, cu])
+    _sig(L, "estl_ubatch_execute", ci, [vp, vp, vp, vp])
+    _sig(L, "estl_ubatch_execute_graph", ci, [vp, vp, vp, vp])
+    _sig(L, "estl_ubatch_describe", ci, [vp, ctypes.c_char_p, ci])
+    _sig(L, "estl_ubatch_destroy", ci, [vp])
     _lib = L
     return L
 
@@ -97,14 +106,15 @@ class Plan:
     """Device plan for `batch` rows of length n (C ABI estl_plan_create)."""
 
     def __init__(self, n: int, batch: int = 1, direction: int = FORWARD, precision: int = FP32,
-                 flags: int = 0, split: list[int] | None = None):
+                 flags: int = 0, split: list[int] | None = None, tiles: list[int] | None = None):
         self.n, self.batch, self.direction, self.precision, self.flags = n, batch, direction, precision, flags
         self._h = ctypes.c_void_p()
         h = ctypes.c_void_p()
         if split:
             arr = (ctypes.c_int * len(split))(*split)
+            tarr = (ctypes.c_int * len(split))(*tiles) if tiles else None
             check(lib().estl_plan_create_split(ctypes.byref(h), n, batch, direction, precision, flags,
-                                               len(split), arr))
+                                               len(split), arr, tarr))
         else:
             check(lib().estl_plan_create(ctypes.byref(h), n, batch, direction, precision, flags))
         self._h = h
@@ -146,11 +156,61 @@ class Plan:
             pass
 
 
+class UnevenBatch:
+    """Transforms of different power-of-two sizes in one launch (C ABI estl_ubatch_*)."""
+
+    def __init__(self, sizes, offsets=None, direction: int = FORWARD, flags: int = 0):
+        n = len(sizes)
+        self.sizes = list(sizes)
+        szs = (ctypes.c_long * n)(*self.sizes)
+        offs = (ctypes.c_long * n)(*offsets) if offsets is not None else None
+        self._h = ctypes.c_void_p()
+        h = ctypes.c_void_p()
+        check(lib().estl_ubatch_create(ctypes.byref(h), n, szs, offs, direction, flags))
+        self._h = h
+        self.info = self.describe()
+
+    def describe(self) -> dict:
+        buf = ctypes.create_string_buffer(1024)
+        check(lib().estl_ubatch_describe(self._h, buf, 1024))
+        return json.loads(buf.value.decode())
+
+    def _check_buffers(self, d_in, d_out):
+        for t in (d_in, d_out):
+            if not isinstance(t, int) and t.numel() < self.info["extent"]:
+                raise ValueError(f"buffer holds {t.numel()} values, the batch needs {self.info['extent']}")
+
+    def execute(self, d_in, d_out, stream=None, graph: bool = False) -> None:
+        self._check_buffers(d_in, d_out)
+        pin = d_in if isinstance(d_in, int) else d_in.data_ptr()
+        pout = d_out if isinstance(d_out, int) else d_out.data_ptr()
+        fn = lib().estl_ubatch_execute_graph if graph else lib().estl_ubatch_execute
+        check(fn(self._h, pin, pout, _stream_ptr(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().estl_ubatch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def fill_uniform(t, seed: int, offset: int = 0, stream=None) -> None:
     """Fill a float32/complex64 device tensor with the seeded uniform[-1,1) stream."""
     import torch
     count = t.numel() * (2 if t.dtype == torch.complex64 else 1)
     check(lib().estl_fill_uniform(t.data_ptr(), count, seed, offset, _stream_ptr(stream)))
+
+
+def tune(n: int, batch: int = 1, direction: int = FORWARD) -> dict:
+    """Autotune the plan for (n, batch, direction); returns every candidate's time."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(lib().estl_tune(n, batch, direction, buf, 1 << 16))
+    return json.loads(buf.value.decode())
 
 
 def max_batch(n: int, precision: int = FP32, budget_bytes: int = 0) -> int:

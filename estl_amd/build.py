@@ -24,15 +24,27 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I", os.path.join(HERE, "..", "include"), "-Wall", "-Wno-unused-function"]
 
-SOURCES = ["kernels.hip", "fft_fp16.hip", "workqueue.hip", "conv.hip", "estl_c.cpp", "fp16_c.cpp",
-           "ubatch_c.cpp", "conv_c.cpp", "server.cpp"]
+# (source, extra defines, object name)
+SOURCES = [("kernels.hip", [], "kernels"),
+           ("pass.hip", ["-DESTL_PASS_DIR=-1"], "pass_fwd"),
+           ("pass.hip", ["-DESTL_PASS_DIR=+1"], "pass_bwd"),
+           ("workqueue.hip", [], "workqueue"),
+           ("fp16.hip", [], "fp16"),
+           ("conv.hip", [], "conv"),
+           ("estl_c.cpp", [], "estl_c"),
+           ("tune.cpp", [], "tune"),
+           ("fp16_c.cpp", [], "fp16_c"),
+           ("ubatch_c.cpp", [], "ubatch_c"),
+           ("conv_c.cpp", [], "conv_c"),
+           ("server.cpp", [], "server")]
 
 
-def _compile(src: str, verbose: bool) -> str:
+def _compile(item, verbose: bool) -> str:
+    src, defines, name = item
     path = os.path.join(CSRC, src)
-    obj = os.path.join(OBJ, src + ".o")
+    obj = os.path.join(OBJ, name + ".o")
     lang = ["-x", "hip"] if src.endswith(".hip") else []
-    cmd = [HIPCC, *COMMON, *lang, "-c", path, "-o", obj]
+    cmd = [HIPCC, *COMMON, *defines, *lang, "-c", path, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -47,7 +59,7 @@ def _compile(src: str, verbose: bool) -> str:
 
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s[0]))]
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB + ".tmp"

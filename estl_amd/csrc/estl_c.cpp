@@ -124,17 +124,33 @@ static int ilog2(long n) {
     return l;
 }
 
-// Split log2n into passes of at most 9 bits (512-point radices keep a 16-column tile
-// inside 64 KiB of LDS), as evenly as possible, larger radices first.
+// Split log2n into passes of at most 9 bits, as evenly as possible, smaller radices
+// first (a 512-point radix still leaves room for a 16-column tile in 64 KiB of LDS;
+// measured on MI355X, the first pass -- the one without pre-twiddles -- is the
+// cheapest place for the smaller radix).
 std::vector<int> default_pass_split(int log2n) {
     int P = (log2n + 8) / 9;
     if (P < 2) P = 2;
     std::vector<int> bits(P, log2n / P);
-    for (int i = 0; i < log2n % P; ++i) bits[i]++;
+    for (int i = 0; i < log2n % P; ++i) bits[P - 1 - i]++;
     return bits;
 }
 
-int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned flags, const std::vector<int> *split) {
+// Columns per workgroup for a pass of radix 2^log2b over `cols` columns in total:
+// as wide as a 2^13-value tile (64 KiB of LDS) allows (wider tiles mean longer
+// contiguous runs per load and store), capped at 64, narrowed while that leaves fewer
+// than 512 tiles to spread over the CUs (small problems are latency-bound).
+int default_pass_tile(int log2b, int log2cpr, long total_cols) {
+    int lt = 13 - log2b;
+    if (lt > kMaxPassCols) lt = kMaxPassCols;
+    if (lt > log2cpr) lt = log2cpr;
+    if (lt < 0) lt = 0;
+    while (lt > 2 && (total_cols >> lt) < 512) --lt;
+    return lt;
+}
+
+int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned flags, const std::vector<int> *split,
+               const std::vector<int> *tiles) {
     if (n < 1 || (n & (n - 1)))
         return set_error(ESTL_ERR_INVALID, "n=%ld is not a power of two", n);
     if (batch < 1) return set_error(ESTL_ERR_INVALID, "batch=%ld must be >= 1", batch);
@@ -167,19 +183,27 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
         total += b;
     }
     if (total != p->log2n) return set_error(ESTL_ERR_INVALID, "pass split does not sum to log2n=%d", p->log2n);
+    if (p->log2n > 30) return set_error(ESTL_ERR_UNSUPPORTED, "n=2^%d exceeds 2^30", p->log2n);
     long A = 1;
     for (size_t q = 0; q < bits.size(); ++q) {
         PassStep st{};
         st.log2b = bits[q];
         const long B = 1L << bits[q];
-        st.args.n = n;
         st.args.batch = batch;
         st.args.A = A;
-        st.args.S = n / (A * B);
+        st.args.log2n = p->log2n;
+        st.args.log2S = ilog2(n / (A * B));
         st.args.scale = (q + 1 == bits.size()) ? p->scale : 1.0f;
-        st.args.xcd_remap = 1;
-        const int T = pass_tile(bits[q]);
-        if ((n / B) % T) return set_error(ESTL_ERR_UNSUPPORTED, "tile %d does not divide %ld columns", T, n / B);
+        st.log2t = default_pass_tile(bits[q], p->log2n - bits[q], batch * (n / B));
+        st.args.xcd_remap = 0;
+        if (tiles && tiles->size() == bits.size()) {
+            const int lt = (*tiles)[q];
+            if (lt < 0 || lt > kMaxPassCols || lt + bits[q] > kMaxPassTileLog2 || lt > p->log2n - bits[q])
+                return set_error(ESTL_ERR_UNSUPPORTED, "tile 2^%d invalid for pass radix 2^%d", lt, bits[q]);
+            st.log2t = lt;
+            // tiles narrower than a 128-byte line share lines with their neighbours
+            st.args.xcd_remap = (lt < 4) ? 1 : 0;
+        }
         int rc = block_table(p->device, bits[q], &st.args.tw_local);
         if (rc) return rc;
         if (A > 1) {
@@ -230,7 +254,7 @@ int execute_plan(Plan *p, const float2 *in, float2 *out, hipStream_t st) {
             }
             const float2 *src = in;
             for (int q = 0; q < P; ++q) {
-                HIP_TRY(launch_pass(p->steps[q].log2b, p->dir, src, dst[q], p->steps[q].args, st));
+                HIP_TRY(launch_pass(p->steps[q].log2b, p->steps[q].log2t, p->dir, src, dst[q], p->steps[q].args, st));
                 src = dst[q];
             }
             return ESTL_OK;
@@ -274,7 +298,7 @@ std::string describe_plan(const Plan *p) {
     for (size_t q = 0; q < p->steps.size(); ++q) {
         const auto &st = p->steps[q];
         snprintf(buf, sizeof buf, "%s{\"radix\": %ld, \"A\": %ld, \"S\": %ld, \"tile\": %d}", q ? ", " : "",
-                 1L << st.log2b, st.args.A, st.args.S, pass_tile(st.log2b));
+                 1L << st.log2b, st.args.A, 1L << st.args.log2S, 1 << st.log2t);
         s += buf;
     }
     snprintf(buf, sizeof buf, "], \"launches\": %zu}", p->steps.size());
@@ -314,7 +338,16 @@ int estl_plan_create(estl_plan_t *plan, long n, long batch, int direction, int p
     if (!plan) return set_error(ESTL_ERR_INVALID, "plan pointer is null");
     *plan = nullptr;
     Plan *p = new Plan();
-    int rc = build_plan(p, n, batch, direction, precision, flags, nullptr);
+    int rc;
+    if ((flags & ESTL_FLAG_AUTOTUNE) && n >= 2 && !(n & (n - 1)) && precision == ESTL_PRECISION_FP32 &&
+        (direction == ESTL_FORWARD || direction == ESTL_BACKWARD) && batch >= 1) {
+        bool rows = false;
+        std::vector<int> split, tiles;
+        rc = tune_plan(n, batch, direction, &rows, &split, &tiles, nullptr);
+        if (!rc) rc = build_plan(p, n, batch, direction, precision, flags, rows ? nullptr : &split, rows ? nullptr : &tiles);
+    } else {
+        rc = build_plan(p, n, batch, direction, precision, flags, nullptr);
+    }
     if (rc) {
         destroy_plan(p);
         return rc;
@@ -324,14 +357,16 @@ int estl_plan_create(estl_plan_t *plan, long n, long batch, int direction, int p
 }
 
 int estl_plan_create_split(estl_plan_t *plan, long n, long batch, int direction, int precision, unsigned flags,
-                           int npasses, const int *log2radices) {
+                           int npasses, const int *log2radices, const int *log2tiles) {
     if (!plan) return set_error(ESTL_ERR_INVALID, "plan pointer is null");
     *plan = nullptr;
     if (npasses < 0 || npasses > 16 || (npasses > 0 && !log2radices))
         return set_error(ESTL_ERR_INVALID, "bad pass split");
     std::vector<int> split(log2radices, log2radices + npasses);
+    std::vector<int> tiles;
+    if (log2tiles && npasses > 0) tiles.assign(log2tiles, log2tiles + npasses);
     Plan *p = new Plan();
-    int rc = build_plan(p, n, batch, direction, precision, flags, &split);
+    int rc = build_plan(p, n, batch, direction, precision, flags, &split, &tiles);
     if (rc) {
         destroy_plan(p);
         return rc;

@@ -71,8 +71,11 @@ struct Lds<kLdsXor> {
 // `lds` is this sequence's LDS slice (Lds<LAYOUT>::size(N) float2); `tw` the size-N table.
 // All threads of the workgroup must call it together (it holds workgroup barriers).
 // If `lds_busy` is set, the slice is still being read on entry and a barrier is placed
-// before the first LDS write.
-template <int LOG2N, int DIR, int LAYOUT = kLdsPad>
+// before the first LDS write.  HOIST_TW lets the compiler keep the twiddles of every pass
+// in registers across calls made from a loop (fastest when the loop runs one size, as
+// in k_fft_rows); callers that dispatch many sizes in one loop pass false so each
+// size's twiddles are reloaded (from L1/L2) instead of all staying live.
+template <int LOG2N, int DIR, int LAYOUT = kLdsPad, bool HOIST_TW = false>
 __device__ __forceinline__ void block_fft(float2 (&v)[BlockGeom<LOG2N>::E], float2 *lds, int tl,
                                           const float2 *__restrict__ tw, bool lds_busy = false) {
     using G = BlockGeom<LOG2N>;
@@ -104,7 +107,7 @@ __device__ __forceinline__ void block_fft(float2 (&v)[BlockGeom<LOG2N>::E], floa
                 float2 u[B];
 #pragma unroll
                 for (int s = 0; s < B; ++s) u[s] = lds[Lds<LAYOUT>::idx(base + S * s)];
-                const float2 *twq = tw + G::twoff(q) + g;
+                const float2 *twq = (HOIST_TW ? tw : opaque(tw)) + G::twoff(q) + g;
 #pragma unroll
                 for (int s = 1; s < B; ++s) u[s] = ctw<DIR>(u[s], twq[(s - 1) * Aq]);
                 Dft<B, DIR>::run(u);

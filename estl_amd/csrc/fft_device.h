@@ -51,6 +51,23 @@ __device__ __forceinline__ void st_stream(float2 *p, float2 x) {
     __builtin_nontemporal_store(v, reinterpret_cast<v2f_t *>(p));
 }
 
+// Returns p unchanged, but hides from the optimiser that it is loop-invariant.  Used on
+// twiddle-table pointers inside loops over rows/items: otherwise every twiddle load of a
+// pass is hoisted out of the loop and ~30 float2 stay live for the whole kernel, which
+// costs occupancy; re-issued per use they hit in L1/L2.
+template <class T>
+__device__ __forceinline__ const T *opaque(const T *p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+// Same for a per-lane value (the thread index): in loops whose body dispatches on a
+// size class, addresses derived from the thread index alone would otherwise be hoisted
+// for every class at once and all stay live.
+__device__ __forceinline__ int opaque_lane(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // ---------------------------------------------------------------- compile-time trig
 namespace ct {
 constexpr double kPi = 3.141592653589793238462643383279502884;

@@ -8,12 +8,12 @@ namespace estl {
 
 // Arguments of one global Stockham pass (k_fft_pass).
 struct PassArgs {
-    long n;                  // row length
     long batch;              // rows
     long A;                  // A_q: product of the radices of the earlier passes
-    long S;                  // n / (A_q B_q)
+    int log2n;               // row length n = 2^log2n (n <= 2^30)
+    int log2S;               // S = n / (A_q B_q) = 2^log2S
     int lobits;              // two-level pre-twiddle split: m = (hi << lobits) | lo
-    int xcd_remap;           // deal consecutive tiles to one XCD
+    int xcd_remap;           // deal consecutive tiles to the same XCD
     float scale;             // applied to the stored values (1 = none)
     const float2 *tw_local;  // block table of the radix-B DFT
     const float2 *tw_lo;     // W_{A B}^{lo}, lo < 2^lobits
@@ -22,16 +22,22 @@ struct PassArgs {
 
 hipError_t launch_rows(int log2n, int dir, const float2 *in, float2 *out, long batch, const float2 *tw, float scale,
                        hipStream_t st);
-hipError_t launch_pass(int log2b, int dir, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st);
+hipError_t launch_pass_fwd(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st);
+hipError_t launch_pass_bwd(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st);
+inline hipError_t launch_pass(int log2b, int log2t, int dir, const float2 *in, float2 *out, const PassArgs &a,
+                              hipStream_t st) {
+    return dir < 0 ? launch_pass_fwd(log2b, log2t, in, out, a, st) : launch_pass_bwd(log2b, log2t, in, out, a, st);
+}
 hipError_t launch_twiddle(float2 *dst, long count, int mode, long A, long mult, long nq, hipStream_t st);
 hipError_t launch_fill_uniform(float *dst, long count, uint64_t seed, uint64_t offset, hipStream_t st);
 
 long rows_twiddle_count(int log2n);
 void rows_pass_params(int log2n, int q, long *A, int *B);
 int rows_pass_count(int log2n);
-int pass_tile(int log2b);
 
 constexpr int kMaxRowsLog2 = 13;   // largest single-launch row length
 constexpr int kMaxPassLog2 = 13;   // largest radix of a global pass
+constexpr int kMaxPassTileLog2 = 14;  // a pass tile holds at most 2^14 values (~139 KiB of LDS)
+constexpr int kMaxPassCols = 6;    // at most 2^6 columns per tile
 
 }  // namespace estl

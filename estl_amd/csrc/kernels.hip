@@ -4,10 +4,7 @@
 //                  and one HBM write per element (replaces the ceil(log2N/4) launches,
 //                  and HBM round trips, of FFT_OpenCL_Contiguous::execute,
 //                  FFT_OpenCL_Contiguous.h:108-124)
-//   k_fft_pass     one global Stockham pass of radix B (B up to 2^13, itself computed
-//                  by block_fft through LDS) for large N: the pass formula of
-//                  FFT_OpenCL_Contiguous.h:167-180 with a tile of T adjacent columns per
-//                  workgroup so loads and stores stay coalesced
+//   (k_fft_pass, the global Stockham pass of multi-pass plans, is in pass.hip)
 //   k_twiddle      twiddle tables evaluated in FP64 on the GPU and rounded to FP32
 //                  (replaces the per-element native_cos/native_sin of
 //                  KernelGenerator.h:48)
@@ -79,7 +76,7 @@ k_fft_rows(const float2 *__restrict__ in, float2 *__restrict__ out, long batch,
 #pragma unroll
                 for (int i = 0; i < E; ++i) vn[i] = ld_stream(in + nr * N + tl + TPT * i);
             }
-            block_fft<LOG2N, DIR, kLdsPad>(v, lds, tl, tw, busy);
+            block_fft<LOG2N, DIR, kLdsPad, true>(v, lds, tl, tw, busy);
             busy = true;
             float2 *dst = out + row * N + tl;
 #pragma unroll
@@ -106,7 +103,10 @@ k_fft_rows(const float2 *__restrict__ in, float2 *__restrict__ out, long batch,
 #pragma unroll
                 for (int i = 0; i < E; ++i) vn[i] = ld_stream(src + TPT * i);
             }
-            block_fft<LOG2N, DIR, kLdsPad>(v, lds + w * SLICE, tl, tw, busy);
+            {
+                const int to = opaque_lane(t);
+                block_fft<LOG2N, DIR, kLdsPad>(v, lds + (to / TPT) * SLICE, to % TPT, tw, busy);
+            }
             busy = true;
             if (row < batch) {
                 float2 *dst = out + row * N + tl;
@@ -116,84 +116,6 @@ k_fft_rows(const float2 *__restrict__ in, float2 *__restrict__ out, long batch,
 #pragma unroll
             for (int i = 0; i < E; ++i) v[i] = vn[i];
         }
-    }
-}
-
-// ------------------------------------------------------------------ global pass
-// One pass q of a multi-pass plan over `batch` rows of length n: radix B = 2^LOG2B,
-// A = A_q, S = n / (A B).  Column col in [0, batch * n / B) is (row r, j in [0, n/B)),
-// g = j / S, z = j % S.  Column c of the tile is handled by threads c + T * tl.
-//   load   x_s = in[r n + S (B g + s) + z] * W_{A B}^{g s}   (two-level table)
-//   DFT_B  through block_fft (LDS)
-//   store  out[r n + j + h n / B] = X_h
-// If S >= T the tile's loads are T contiguous values per s (direct, coalesced); else the
-// tile's input is one contiguous block of T*B values, staged through LDS.
-template <int LOG2B, int DIR, int T>
-__global__ void __launch_bounds__(BlockGeom<LOG2B>::TPT * T)
-k_fft_pass(const float2 *__restrict__ in, float2 *__restrict__ out, PassArgs a) {
-    using G = BlockGeom<LOG2B>;
-    constexpr int B = G::N, E = G::E, TPT = G::TPT;
-    constexpr int LDSP = Lds<kLdsPad>::size(B);  // one column's block_fft slice (also stages loads)
-    __shared__ float2 lds[T * LDSP];
-    const int t = threadIdx.x;
-    const int c = t % T, tl = t / T;
-    // XCD-aware tile order: consecutive tiles land on the same XCD (blocks are dealt
-    // round-robin to the 8 XCDs), so tiles that share cache lines share an L2.
-    long blk = blockIdx.x;
-    if (a.xcd_remap) {
-        const long nb = gridDim.x;
-        const long per = nb >> 3;
-        if ((nb & 7) == 0) blk = (blk & 7) * per + (blk >> 3);
-    }
-    const long cols_per_row = a.n >> LOG2B;
-    const long col0 = blk * T;
-    const long r = col0 / cols_per_row;
-    const long j0 = col0 - r * cols_per_row;
-    const long j = j0 + c;
-    const long S = a.S;
-    const long g = j / S, z = j - (j / S) * S;
-    const float2 *src = in + r * a.n;
-    float2 v[E];
-    bool lds_busy = false;
-    if (S >= T) {
-        const float2 *p = src + S * B * g + z;
-#pragma unroll
-        for (int i = 0; i < E; ++i) v[i] = p[S * (tl + TPT * i)];
-    } else {
-        // contiguous block of T*B values starting at B*j0: value o -> column
-        // (o / (S B)) * S + o % S, element (o % (S B)) / S
-        const float2 *p = src + (long)B * j0;
-        constexpr int NT = T * TPT;
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const int o = t + NT * k;
-            const int gl = o / (int(S) * B), rem = o - gl * (int(S) * B);
-            const int s = rem / int(S), zz = rem - s * int(S);
-            lds[(gl * int(S) + zz) * LDSP + Lds<kLdsPad>::idx(s)] = p[o];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < E; ++i) v[i] = lds[c * LDSP + Lds<kLdsPad>::idx(tl + TPT * i)];
-        lds_busy = true;
-    }
-    if (a.A > 1) {
-        const long lomask = (1L << a.lobits) - 1;
-#pragma unroll
-        for (int i = 0; i < E; ++i) {
-            const long m = g * (long)(tl + TPT * i);
-            const float2 w = cmul(a.tw_lo[m & lomask], a.tw_hi[m >> a.lobits]);
-            v[i] = ctw<DIR>(v[i], w);
-        }
-    }
-    block_fft<LOG2B, DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, lds_busy);
-    float2 *dst = out + r * a.n + j;
-    const long stride = cols_per_row;
-    if (a.scale != 1.0f) {
-#pragma unroll
-        for (int i = 0; i < E; ++i) dst[stride * (tl + TPT * i)] = cscale(v[i], a.scale);
-    } else {
-#pragma unroll
-        for (int i = 0; i < E; ++i) dst[stride * (tl + TPT * i)] = v[i];
     }
 }
 
@@ -270,42 +192,6 @@ void rows_pass_params(int log2n, int q, long *A, int *B) {
     *B = (q < nfull) ? 16 : (1 << rem);
 }
 int rows_pass_count(int log2n) { return log2n / 4 + ((log2n % 4) ? 1 : 0); }
-
-int pass_tile(int log2b) {
-    // T adjacent columns per workgroup: 16 (128-byte store segments) while the tile
-    // fits 64 KiB of LDS values, fewer for the largest radices
-    int t = 8192 >> log2b;
-    if (t > 16) t = 16;
-    if (t < 1) t = 1;
-    return t;
-}
-
-template <int LOG2B, int DIR>
-static hipError_t launch_pass_t(const float2 *in, float2 *out, const PassArgs &a, hipStream_t st) {
-    using G = BlockGeom<LOG2B>;
-    constexpr int T = (8192 >> LOG2B) > 16 ? 16 : ((8192 >> LOG2B) < 1 ? 1 : (8192 >> LOG2B));
-    const long cols = a.batch * (a.n >> LOG2B);
-    if (cols % T) return hipErrorInvalidValue;
-    const long blocks = cols / T;
-    hipLaunchKernelGGL((k_fft_pass<LOG2B, DIR, T>), dim3((unsigned)blocks), dim3(G::TPT * T), 0, st, in, out, a);
-    return hipGetLastError();
-}
-
-template <int DIR>
-static hipError_t launch_pass_d(int log2b, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st) {
-    switch (log2b) {
-#define PASS_CASE(L) \
-    case L: return launch_pass_t<L, DIR>(in, out, a, st);
-        PASS_CASE(4) PASS_CASE(5) PASS_CASE(6) PASS_CASE(7) PASS_CASE(8) PASS_CASE(9) PASS_CASE(10)
-        PASS_CASE(11) PASS_CASE(12) PASS_CASE(13)
-#undef PASS_CASE
-        default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_pass(int log2b, int dir, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st) {
-    return dir < 0 ? launch_pass_d<-1>(log2b, in, out, a, st) : launch_pass_d<+1>(log2b, in, out, a, st);
-}
 
 hipError_t launch_twiddle(float2 *dst, long count, int mode, long A, long mult, long nq, hipStream_t st) {
     if (count <= 0) return hipSuccess;

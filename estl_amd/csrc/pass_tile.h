@@ -1,0 +1,96 @@
+// pass_tile.h -- one tile of a global Stockham pass, shared by k_fft_pass (one tile
+// per workgroup) and the uneven-batch work-queue kernel (many tiles per workgroup).
+//
+// Pass q over `batch` rows of length n = 2^log2n with radix B = 2^LOG2B, A = A_q,
+// S = n / (A B) = 2^log2S follows FFT_OpenCL_Contiguous.h:167-180: column
+// j in [0, n/B) of a row has g = j / S, z = j mod S and
+//   load   x_s = in[S (B g + s) + z] * W_{A B}^{g s}      s in [0, B)
+//   DFT_B  of x (block_fft: registers + LDS exchanges)
+//   store  out[j + h n / B] = X_h                          h in [0, B)
+// Where the reference launches one kernel per radix-16 pass, a pass here is a radix of
+// up to 2^13 computed inside one workgroup.  A tile is T = 2^LOG2T adjacent columns
+// (thread c + T * tl serves column c), so every load and store moves T consecutive
+// complex values: when S >= T the loads read T contiguous values per s; otherwise the
+// tile's input is one contiguous block of T B values, staged through LDS.  Offsets inside
+// a row are 32-bit (n <= 2^30); input and output are streamed.
+#pragma once
+
+#include "fft_block.h"
+#include "internal.h"
+
+namespace estl {
+
+template <int LOG2B, int LOG2T>
+struct PassTile {
+    static constexpr int B = 1 << LOG2B, T = 1 << LOG2T;
+    // Adjacent lanes serve adjacent columns, so column slices are strided by an odd
+    // number of float2: 16 lanes at the same offset in 16 columns then land on 16
+    // distinct 8-byte bank pairs (all 32 banks).
+    static constexpr int ldsp = Lds<kLdsPad>::size(B) | 1;
+    static constexpr int lds_size = T * ldsp;
+    static constexpr int threads = BlockGeom<LOG2B>::TPT << LOG2T;
+};
+
+// Computes tile `tile` of the pass described by `a`.  All threads of the workgroup
+// (PassTile::threads of them) must call it together.  `lds` holds PassTile::lds_size
+// float2; if `lds_busy`, a previous user of it may still be reading and a barrier
+// comes first.
+template <int LOG2B, int DIR, int LOG2T>
+__device__ __forceinline__ void pass_tile(const float2 *__restrict__ in, float2 *__restrict__ out, const PassArgs &a,
+                                          long tile, float2 *lds, bool lds_busy) {
+    using G = BlockGeom<LOG2B>;
+    constexpr int E = G::E, TPT = G::TPT, T = 1 << LOG2T, NT = T * TPT;
+    constexpr int LDSP = PassTile<LOG2B, LOG2T>::ldsp;
+    const int t = opaque_lane(threadIdx.x), c = t & (T - 1), tl = t >> LOG2T;
+    const int ls = a.log2S;
+    const int lcpr = a.log2n - LOG2B;  // columns per row = 2^lcpr
+    const long row = tile >> (lcpr - LOG2T);
+    const int j0 = int(tile - (row << (lcpr - LOG2T))) << LOG2T;
+    const int j = j0 + c;
+    const int g = j >> ls, z = j & ((1 << ls) - 1);
+    const float2 *src = in + (row << a.log2n);
+    float2 v[E];
+    bool busy = lds_busy;
+    if (ls >= LOG2T) {
+        const float2 *p = src + ((g << (ls + LOG2B)) + z);
+#pragma unroll
+        for (int i = 0; i < E; ++i) v[i] = ld_stream(p + ((tl + TPT * i) << ls));
+    } else {
+        // contiguous block of T*B values starting at B*j0: value o belongs to column
+        // (o / (S B)) S + o mod S, element (o mod S B) / S
+        const float2 *p = src + (j0 << LOG2B);
+        const int sbmask = (1 << (ls + LOG2B)) - 1, smask = (1 << ls) - 1;
+        float2 r[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) r[k] = ld_stream(p + t + NT * k);
+        if (busy) __syncthreads();
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int o = t + NT * k;
+            const int gl = o >> (ls + LOG2B), rem = o & sbmask;
+            const int s = rem >> ls, zz = rem & smask;
+            lds[((gl << ls) + zz) * LDSP + Lds<kLdsPad>::idx(s)] = r[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < E; ++i) v[i] = lds[c * LDSP + Lds<kLdsPad>::idx(tl + TPT * i)];
+        busy = true;
+    }
+    if (a.A > 1) {
+        // W_{A B}^{g s} = W^{lo} W^{hi << lobits}, m = g s < A B (two exactly rounded
+        // table values, one complex product)
+        const unsigned lomask = (1u << a.lobits) - 1;
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const unsigned m = unsigned(g) * unsigned(tl + TPT * i);
+            const float2 w = cmul(a.tw_lo[m & lomask], a.tw_hi[m >> a.lobits]);
+            v[i] = ctw<DIR>(v[i], w);
+        }
+    }
+    block_fft<LOG2B, DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, busy);
+    float2 *dst = out + (row << a.log2n) + j;
+#pragma unroll
+    for (int i = 0; i < E; ++i) st_stream(dst + ((tl + TPT * i) << lcpr), cscale(v[i], a.scale));
+}
+
+}  // namespace estl

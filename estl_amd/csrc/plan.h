@@ -12,7 +12,8 @@
 namespace estl {
 
 struct PassStep {
-    int log2b;
+    int log2b;   // radix 2^log2b
+    int log2t;   // 2^log2t adjacent columns per workgroup
     PassArgs args;
 };
 
@@ -38,10 +39,13 @@ int current_device();
 int block_table(int device, int log2n, const float2 **out);
 int linear_table(int device, long count, long mult, long nq, const float2 **out);
 std::vector<int> default_pass_split(int log2n);
-int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned flags, const std::vector<int> *split);
+int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned flags, const std::vector<int> *split,
+               const std::vector<int> *tiles = nullptr);
 int execute_plan(Plan *p, const float2 *in, float2 *out, hipStream_t st);
 void destroy_plan(Plan *p);
 std::string describe_plan(const Plan *p);
 int plan_launches(const Plan *p);
+int tune_plan(long n, long batch, int dir, bool *rows, std::vector<int> *split, std::vector<int> *tiles,
+              std::string *report);
 
 }  // namespace estl

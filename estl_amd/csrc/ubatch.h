@@ -1,0 +1,45 @@
+// ubatch.h -- descriptors of the uneven-size batch (work-queue kernel k_ubatch).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace estl {
+
+constexpr int kUMaxLog2 = 20;       // largest transform of an uneven batch
+constexpr int kUMaxSmallLog2 = 12;  // up to here a work item is a group of rows
+
+// One work item: `count` transforms of size 2^log2n, the `first`-th.. of the
+// size-sorted offset list (count = 1 for large sizes).
+struct UItem {
+    int log2n;
+    int count;
+    int first;
+    int pad;
+};
+
+// Device tables of every size class.
+struct UTables {
+    const float2 *block[kUMaxSmallLog2 + 1];  // BlockGeom tables, log2 0..12
+    const float2 *lo[kUMaxLog2 + 1];          // pre-twiddle tables of the second pass
+    const float2 *hi[kUMaxLog2 + 1];
+    int lobits[kUMaxLog2 + 1];
+};
+
+struct UArgs {
+    const UItem *items;
+    int n_items;
+    const long *offs;     // element offset of each transform, size-sorted
+    const float2 *in;
+    float2 *out;
+    float2 *scratch;      // pass-1 output of large transforms (same offsets)
+    int *counters;        // [0] ticket, [1] workgroups done (both zero between launches)
+    float scale;
+    UTables tw;
+};
+
+hipError_t launch_ubatch(int dir, const UArgs &a, long blocks, hipStream_t st);
+long ubatch_resident_blocks();
+int ubatch_small_rows_per_item(int log2n);
+int ubatch_split_log2(int log2n);
+
+}  // namespace estl

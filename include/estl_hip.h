@@ -45,8 +45,10 @@ extern "C" {
 #define ESTL_PRECISION_FP16 1      /* half2 storage + per-row FP32 scale, FP32 arithmetic      */
 
 #define ESTL_FLAG_NORMALIZE 0x1u   /* multiply the result by 1/n                               */
+#define ESTL_FLAG_AUTOTUNE 0x2u    /* pick the plan by timing candidates on the device (cached) */
 
 typedef struct estl_plan_s *estl_plan_t;
+typedef struct estl_ubatch_s *estl_ubatch_t;
 
 /* Message of the calling thread's last failure ("" if none). */
 const char *estl_last_error(void);
@@ -68,12 +70,21 @@ int estl_init(int device);
 int estl_plan_create(estl_plan_t *plan, long n, long batch, int direction, int precision, unsigned flags);
 
 /* Same as estl_plan_create with an explicit pass split: npasses global passes of radix
- * 2^log2radices[q] (each in [4, 13], summing to log2 n).  npasses = 0 picks the
- * default plan.  The reference fixes this split to radix-16 passes with the remainder
- * last (BGs, FFT_OpenCL_Contiguous.h:55-60); here it is the plan space the autotuner
- * searches. */
+ * 2^log2radices[q] (each in [4, 13], summing to log2 n), each computed by workgroups
+ * owning 2^log2tiles[q] adjacent columns (log2tiles may be NULL: default widths).
+ * npasses = 0 picks the default plan.  The reference fixes the split to radix-16
+ * passes with the remainder last (BGs, FFT_OpenCL_Contiguous.h:55-60); here it is the
+ * plan space the autotuner searches. */
 int estl_plan_create_split(estl_plan_t *plan, long n, long batch, int direction, int precision, unsigned flags,
-                           int npasses, const int *log2radices);
+                           int npasses, const int *log2radices, const int *log2tiles);
+
+/* Autotune the plan for (n, batch, direction) on the current device: time the single-
+ * launch kernel (n <= 2^13) and the pass splits near the balanced one, keep the fastest
+ * (used by later estl_plan_create calls with ESTL_FLAG_AUTOTUNE; persisted to the file
+ * named by ESTL_TUNE_CACHE if set).  Writes a JSON report of every candidate's time into
+ * report (may be NULL) and returns its length.  Replaces the fixed radix-16 choice of
+ * FFT_OpenCL_Contiguous.h:55-60. */
+int estl_tune(long n, long batch, int direction, char *report, int len);
 
 /* Destroy a plan (FFT_OpenCL_Contiguous::~FFT_OpenCL_Contiguous, FFT_OpenCL_Contiguous.h:186-191). */
 int estl_plan_destroy(estl_plan_t plan);
@@ -111,6 +122,27 @@ int estl_plan_launches(estl_plan_t plan);
  * value i = ((u >> 40) * 2^-23) - 1, u = splitmix64(splitmix64(seed) + offset + i).
  * No reference counterpart (the reference seeds rand() with time(NULL), tests/Test.cpp:189). */
 int estl_fill_uniform(void *d_dst, long count, uint64_t seed, uint64_t offset, void *stream);
+
+/* ---------------------------------------------------------------- uneven batches
+ * One launch computes `count` transforms of different power-of-two sizes (2 .. 2^20),
+ * transform i of sizes[i] complex64 values at element offset offsets[i] of the in and
+ * out buffers (offsets NULL: packed back to back).  No reference counterpart: the
+ * reference batches only equal sizes (batchCount, FFT.h:25-27) and needs one FFT object,
+ * program and launch sequence per size (tests/Test.cpp:48-53).  Transforms must not
+ * overlap; the buffers must hold the extent estl_ubatch_describe reports. */
+int estl_ubatch_create(estl_ubatch_t *ub, int count, const long *sizes, const long *offsets, int direction,
+                       unsigned flags);
+
+/* One launch of the work-queue kernel on `stream` (asynchronous). */
+int estl_ubatch_execute(estl_ubatch_t ub, const void *d_in, void *d_out, void *stream);
+
+/* The same launch captured once in a HIP graph per (d_in, d_out, stream) and replayed. */
+int estl_ubatch_execute_graph(estl_ubatch_t ub, const void *d_in, void *d_out, void *stream);
+
+/* JSON description (transforms, work items, workgroups, element extent) into buf. */
+int estl_ubatch_describe(estl_ubatch_t ub, char *buf, int len);
+
+int estl_ubatch_destroy(estl_ubatch_t ub);
 
 #ifdef __cplusplus
 }
