@@ -138,6 +138,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-size batch leg")
     ap.add_argument("--percall-rows", type=int, default=BATCH)
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the step's two launches as one HIP graph (measured: no gain over eager launches)")
     ap.add_argument("--counters", action="store_true",
                     help="counter-collection mode for rocprofv3 --pmc: 3 forward launches of the "
                          "batched plan and 3 plain 512 MiB device copies (the calibration kernel), nothing else")
@@ -185,12 +187,22 @@ def main() -> int:
     for _ in range(args.warmup):
         step()
     stream.synchronize()
+    run_step = step
+    if args.graph:
+        # the step's two launches captured once and replayed as one graph (no memset or
+        # allocation is recorded: the plans' kernels need neither)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=stream):
+            step()
+        graph.replay()
+        stream.synchronize()
+        run_step = graph.replay
 
     # timed region: exactly K steps, synchronised on both sides
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        run_step()
     stream.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -281,7 +293,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "batched C2C fp32 forward+backward round trip, N=4096, batch 16384, "
                                "uniform[-1,1) seed 1, HBM-resident",
-                   "n": N, "batch": BATCH, "seed": SEED, "plan": plan_desc},
+                   "n": N, "batch": BATCH, "seed": SEED, "plan": plan_desc, "hip_graph": bool(args.graph)},
         "hbm_frac": round(value / HBM_PEAK_GBS, 4),
         "gflops_5nlog2n": round(2 * 5.0 * N * math.log2(N) * BATCH / (wall / args.steps) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

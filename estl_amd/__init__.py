@@ -24,6 +24,9 @@ FP32 = 0
 FP16 = 1
 FLAG_NORMALIZE = 0x1
 FLAG_AUTOTUNE = 0x2
+CONV_CORRELATE = 0x10
+CONV_NORMALIZE_KERNEL = 0x20
+CONV_CENTER = 0x40
 
 OK = 0
 ERR_INVALID = -1
@@ -79,6 +82,11 @@ def lib():
     _sig(L, "estl_ubatch_execute_graph", ci, [vp, vp, vp, vp])
     _sig(L, "estl_ubatch_describe", ci, [vp, ctypes.c_char_p, ci])
     _sig(L, "estl_ubatch_destroy", ci, [vp])
+    _sig(L, "estl_conv_create", ci, [ctypes.POINTER(vp), cl, cl, ci, cu])
+    _sig(L, "estl_conv_set_filter", ci, [vp, vp, cl, vp])
+    _sig(L, "estl_conv_execute", ci, [vp, vp, vp, vp])
+    _sig(L, "estl_conv_workspace_bytes", ll, [vp])
+    _sig(L, "estl_conv_destroy", ci, [vp])
     _lib = L
     return L
 
@@ -190,6 +198,44 @@ class UnevenBatch:
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
             lib().estl_ubatch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Convolution:
+    """Batched circular convolution of real rows (C ABI estl_conv_*)."""
+
+    def __init__(self, n: int, rows: int, flags: int = 0, precision: int = FP32):
+        self.n, self.rows, self.flags = n, rows, flags
+        self._h = ctypes.c_void_p()
+        h = ctypes.c_void_p()
+        check(lib().estl_conv_create(ctypes.byref(h), n, rows, precision, flags))
+        self._h = h
+
+    def set_filter(self, d_h, k: int | None = None, stream=None) -> None:
+        p = d_h if isinstance(d_h, int) else d_h.data_ptr()
+        k = k if k is not None else d_h.numel()
+        check(lib().estl_conv_set_filter(self._h, p, k, _stream_ptr(stream)))
+
+    def execute(self, d_x, d_y, stream=None) -> None:
+        for t in (d_x, d_y):
+            if not isinstance(t, int) and t.numel() < self.n * self.rows:
+                raise ValueError("buffer smaller than rows * n")
+        px = d_x if isinstance(d_x, int) else d_x.data_ptr()
+        py = d_y if isinstance(d_y, int) else d_y.data_ptr()
+        check(lib().estl_conv_execute(self._h, px, py, _stream_ptr(stream)))
+
+    def workspace_bytes(self) -> int:
+        return check(lib().estl_conv_workspace_bytes(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().estl_conv_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

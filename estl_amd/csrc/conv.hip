@@ -1,0 +1,140 @@
+// conv.hip -- the transform-multiply-inverse pipeline of Convolution.h:95-148, batched
+// over real 1-D rows and fused on the device.
+//
+// The reference pads, transforms data and kernel with FFT2D (one FFT object and one
+// host round trip per row and column, FFT.h:65-81), multiplies on the host
+// (Convolution.h:125-137), transforms back and scales by 1/(K^[0] W H)
+// (Convolution.h:141-143).  Here:
+//   * k_conv_fold   builds the length-n filter: h folded modulo n (a filter longer than
+//                   the signal wraps, the circular definition), optionally rotated so
+//                   its centre (K + 1)/2 lands on index 0 (the reference's kernel shift,
+//                   Convolution.h:113-120); the spectrum is then one forward FFT, kept
+//                   resident;
+//   * k_conv_rows   for n <= 2^13: two real rows enter as one complex row (re = row 2p,
+//                   im = row 2p + 1 -- the filter is real, so the two convolutions stay
+//                   separate), forward FFT, multiply by the spectrum (its conjugate for
+//                   correlation, Convolution.h:125-131), inverse FFT, scale, and the
+//                   re / im parts go back to the two rows: one HBM read and one write
+//                   per value, one launch for the whole batch;
+//   * larger n runs the same pipeline as global passes with fused I/O (pass_tile.h).
+#include "fft_block.h"
+#include "internal.h"
+
+namespace estl {
+
+__global__ void k_conv_fold(const float *__restrict__ h, long k, long n, long shift, float2 *__restrict__ hp) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // hp[i] = sum of h[t] over t with (t - shift) = i (mod n)
+    double acc = 0.0;
+    long t = (i + shift) % n;
+    for (; t < k; t += n) acc += (double)h[t];
+    hp[i] = make_float2((float)acc, 0.0f);
+}
+
+// spectrum /= Re spectrum[0] (the kernel sum), read from a copy `dc` taken beforehand
+__global__ void k_spec_normalize(float2 *__restrict__ spec, long n, const float2 *__restrict__ dc) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float s = 1.0f / dc->x;
+    spec[i] = make_float2(spec[i].x * s, spec[i].y * s);
+}
+
+hipError_t launch_spec_normalize(float2 *spec, long n, const float2 *dc, hipStream_t st) {
+    hipLaunchKernelGGL(k_spec_normalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, spec, n, dc);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv_fold(const float *h, long k, long n, long shift, float2 *hp, hipStream_t st) {
+    const long blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_conv_fold, dim3((unsigned)blocks), dim3(256), 0, st, h, k, n, shift, hp);
+    return hipGetLastError();
+}
+
+// Persistent over pairs of real rows (TPW pairs per workgroup), like k_fft_rows.
+template <int LOG2N, int TPW, bool CORR>
+__global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT * TPW)
+k_conv_rows(const float *__restrict__ x, float *__restrict__ y, long rows, const float2 *__restrict__ spec,
+            const float2 *__restrict__ tw, float scale) {
+    using G = BlockGeom<LOG2N>;
+    constexpr int N = G::N, E = G::E, TPT = G::TPT;
+    constexpr int SLICE = G::uses_lds ? Lds<kLdsPad>::size(N) : 0;
+    constexpr int LDS = G::uses_lds ? TPW * SLICE : 1;
+    __shared__ float2 lds[LDS];
+    const long pairs = (rows + 1) / 2;
+    const long groups = (pairs + TPW - 1) / TPW;
+    for (long grp = blockIdx.x; grp < groups; grp += gridDim.x) {  // uniform across the workgroup
+        const int t = opaque_lane(threadIdx.x);
+        const int w = t / TPT, tl = t % TPT;
+        const long pr = grp * TPW + w;
+        const bool active = pr < pairs;
+        const long r1 = active ? 2 * pr : 0;
+        const bool two = active && r1 + 1 < rows;
+        const float *x1 = x + r1 * N + tl;
+        float2 v[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            v[i] = make_float2(ld_stream_f(x1 + TPT * i), two ? ld_stream_f(x1 + N + TPT * i) : 0.0f);
+        block_fft<LOG2N, -1, kLdsPad>(v, lds + w * SLICE, tl, tw, true);
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const float2 hk = spec[tl + TPT * i];
+            v[i] = CORR ? cmulc(v[i], hk) : cmul(v[i], hk);
+        }
+        block_fft<LOG2N, +1, kLdsPad>(v, lds + w * SLICE, tl, tw, true);
+        if (active) {
+            float *y1 = y + r1 * N + tl;
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                st_stream_f(y1 + TPT * i, v[i].x * scale);
+                if (two) st_stream_f(y1 + N + TPT * i, v[i].y * scale);
+            }
+        }
+    }
+}
+
+template <class K>
+static long conv_resident(K kernel, int threads) {
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    return (long)per_cu * cus;
+}
+
+template <int LOG2N, bool CORR>
+static hipError_t launch_conv_rows_t(const float *x, float *y, long rows, const float2 *spec, const float2 *tw,
+                                     float scale, hipStream_t st) {
+    using G = BlockGeom<LOG2N>;
+    constexpr int TPW = (G::TPT >= 256) ? 1 : 256 / G::TPT;
+    auto kernel = k_conv_rows<LOG2N, TPW, CORR>;
+    static long resident = 0;  // per process; one device per process on this path
+    if (!resident) resident = conv_resident(kernel, G::TPT * TPW);
+    const long pairs = (rows + 1) / 2;
+    long blocks = (pairs + TPW - 1) / TPW;
+    if (blocks > 4 * resident) blocks = 4 * resident;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(G::TPT * TPW), 0, st, x, y, rows, spec, tw, scale);
+    return hipGetLastError();
+}
+
+template <bool CORR>
+static hipError_t launch_conv_rows_c(int log2n, const float *x, float *y, long rows, const float2 *spec,
+                                     const float2 *tw, float scale, hipStream_t st) {
+    switch (log2n) {
+#define CONV_CASE(L) \
+    case L: return launch_conv_rows_t<L, CORR>(x, y, rows, spec, tw, scale, st);
+        CONV_CASE(1) CONV_CASE(2) CONV_CASE(3) CONV_CASE(4) CONV_CASE(5) CONV_CASE(6) CONV_CASE(7) CONV_CASE(8)
+        CONV_CASE(9) CONV_CASE(10) CONV_CASE(11) CONV_CASE(12) CONV_CASE(13)
+#undef CONV_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_conv_rows(int log2n, bool corr, const float *x, float *y, long rows, const float2 *spec,
+                            const float2 *tw, float scale, hipStream_t st) {
+    return corr ? launch_conv_rows_c<true>(log2n, x, y, rows, spec, tw, scale, st)
+                : launch_conv_rows_c<false>(log2n, x, y, rows, spec, tw, scale, st);
+}
+
+}  // namespace estl

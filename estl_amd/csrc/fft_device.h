@@ -50,6 +50,8 @@ __device__ __forceinline__ void st_stream(float2 *p, float2 x) {
     v2f_t v = {x.x, x.y};
     __builtin_nontemporal_store(v, reinterpret_cast<v2f_t *>(p));
 }
+__device__ __forceinline__ float ld_stream_f(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_stream_f(float *p, float x) { __builtin_nontemporal_store(x, p); }
 
 // Returns p unchanged, but hides from the optimiser that it is loop-invariant.  Used on
 // twiddle-table pointers inside loops over rows/items: otherwise every twiddle load of a

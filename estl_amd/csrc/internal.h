@@ -18,6 +18,15 @@ struct PassArgs {
     const float2 *tw_local;  // block table of the radix-B DFT
     const float2 *tw_lo;     // W_{A B}^{lo}, lo < 2^lobits
     const float2 *tw_hi;     // W_{A B}^{hi << lobits}
+    // I/O modes of the convolution pipeline (0 = plain complex rows in and out):
+    //   io_in  = 1: complex row r is the real pair (rin row 2r, rin row 2r + 1)
+    //   io_out = 1: multiply the (natural-order) result by spectrum[k] (conj if corr)
+    //   io_out = 2: write re / im of the scaled result to rout rows 2r, 2r + 1
+    int io_in, io_out, corr;
+    long real_rows;          // real rows behind the pairs (the last pair may be single)
+    const float *rin;
+    float *rout;
+    const float2 *spectrum;
 };
 
 hipError_t launch_rows(int log2n, int dir, const float2 *in, float2 *out, long batch, const float2 *tw, float scale,

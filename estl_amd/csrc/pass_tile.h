@@ -51,7 +51,18 @@ __device__ __forceinline__ void pass_tile(const float2 *__restrict__ in, float2 
     const float2 *src = in + (row << a.log2n);
     float2 v[E];
     bool busy = lds_busy;
-    if (ls >= LOG2T) {
+    if (a.io_in == 1) {
+        // real pair: rows 2 row and 2 row + 1 of rin (the second absent past the end);
+        // only the first pass reads pairs and its plans keep S >= T
+        const float *x1 = a.rin + ((2 * row) << a.log2n);
+        const bool two = 2 * row + 1 < a.real_rows;
+        const int base = (g << (ls + LOG2B)) + z;
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int o = base + ((tl + TPT * i) << ls);
+            v[i] = make_float2(ld_stream_f(x1 + o), two ? ld_stream_f(x1 + (1L << a.log2n) + o) : 0.0f);
+        }
+    } else if (ls >= LOG2T) {
         const float2 *p = src + ((g << (ls + LOG2B)) + z);
 #pragma unroll
         for (int i = 0; i < E; ++i) v[i] = ld_stream(p + ((tl + TPT * i) << ls));
@@ -88,6 +99,25 @@ __device__ __forceinline__ void pass_tile(const float2 *__restrict__ in, float2 
         }
     }
     block_fft<LOG2B, DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, busy);
+    if (a.io_out == 2) {
+        float *y1 = a.rout + ((2 * row) << a.log2n) + j;
+        const bool two = 2 * row + 1 < a.real_rows;
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int o = (tl + TPT * i) << lcpr;
+            st_stream_f(y1 + o, v[i].x * a.scale);
+            if (two) st_stream_f(y1 + (1L << a.log2n) + o, v[i].y * a.scale);
+        }
+        return;
+    }
+    if (a.io_out == 1) {
+        // the last forward pass writes natural order: output h of column j is bin j + h n/B
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const float2 hk = a.spectrum[j + ((tl + TPT * i) << lcpr)];
+            v[i] = a.corr ? cmulc(v[i], hk) : cmul(v[i], hk);
+        }
+    }
     float2 *dst = out + (row << a.log2n) + j;
 #pragma unroll
     for (int i = 0; i < E; ++i) st_stream(dst + ((tl + TPT * i) << lcpr), cscale(v[i], a.scale));

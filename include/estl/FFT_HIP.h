@@ -1,0 +1,81 @@
+// FFT_HIP.h -- the gfx950 backend behind the reference's FFT<D> interface.
+//
+// Drop-in beside FFT_FFTW3 / FFT_OpenCL_Contiguous: same constructor (n, forward,
+// batchCount), static getName() and getMaxBatchSize(n) (what FFTFactorySpecific calls,
+// FFT.h:46-52), interleaved complex64 host arrays through SplitInterleavedDataInterface
+// (planar = false), exactly the data interface of FFT_FFTW3 (FFT_FFTW3.h:24), and a
+// synchronous execute() (copy in, compute, copy out, wait) like
+// FFT_OpenCL_Contiguous::execute (FFT_OpenCL_Contiguous.h:97-142).  The work goes
+// through the C ABI of estl_hip.h; nothing of HIP leaks into this header.
+//
+// Semantics are FFTW's (FFT_FFTW3.h:26-28): forward = exp(-2 pi i t k / n), backward =
+// exp(+...), both unnormalised, applied to every batch row.  Setup failures (size not
+// a power of two, no device, out of memory) throw std::runtime_error; a failure inside
+// execute() is printed and leaves the output untouched, the reference's behaviour
+// (FFT_OpenCL_Contiguous.h:120-122).
+#ifndef ESTL_HIP_FFT_HIP_H
+#define ESTL_HIP_FFT_HIP_H
+
+#include <climits>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "../estl_hip.h"
+#include "FFT.h"
+
+template <class D>
+class FFT_HIP;
+
+template <>
+class FFT_HIP<float> : public FFT<float> {
+  public:
+    typedef SplitInterleavedDataInterface<float> DataInterfaceType;
+
+    static std::string getName() { return "HIP FFT (gfx950)"; }
+
+    // largest batch of length-n rows that fits the device's free memory (64-bit query;
+    // the reference's int query overflows on large devices, uOpenCL.h:323-337)
+    static int getMaxBatchSize(int n) {
+        const long long b = estl_max_batch(n, ESTL_PRECISION_FP32, 0);
+        if (b < 1) return 1;
+        return b > INT_MAX ? INT_MAX : int(b);
+    }
+
+    FFT_HIP(int iSize, bool iForward = true, int iBatchCount = 1, unsigned flags = 0)
+        : FFT<float>(iSize, iBatchCount), forward(iForward), plan(nullptr) {
+        this->dataInterface = arrays = new DataInterfaceType(iSize * iBatchCount, false);
+        const int rc = estl_plan_create(&plan, iSize, iBatchCount, iForward ? ESTL_FORWARD : ESTL_BACKWARD,
+                                        ESTL_PRECISION_FP32, flags);
+        if (rc != ESTL_OK) throw std::runtime_error(std::string("FFT_HIP: ") + estl_last_error());
+    }
+
+    virtual ~FFT_HIP() { estl_plan_destroy(plan); }
+
+    virtual void execute() {
+        const int rc = estl_execute_host(plan, arrays->in->getData(), arrays->out->getData());
+        if (rc != ESTL_OK) printf("FFT_HIP::execute failed: %s\n", estl_last_error());
+    }
+
+    // profiling surface in the spirit of the reference's per-kernel report
+    // (FFT_OpenCL_Contiguous.h:94-95, tests/Test.cpp:86-100)
+    int getKernelCount() const { return estl_plan_launches(plan); }
+    double getTotalComputationFlops() const {
+        int l = 0;
+        while ((1 << l) < size) ++l;
+        return 5.0 * size * l * batchCount;
+    }
+    std::string describe() const {
+        char buf[2048];
+        estl_plan_describe(plan, buf, sizeof buf);
+        return buf;
+    }
+    estl_plan_t getPlan() const { return plan; }
+
+  private:
+    bool forward;
+    DataInterfaceType *arrays;
+    estl_plan_t plan;
+};
+
+#endif  // ESTL_HIP_FFT_HIP_H

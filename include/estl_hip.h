@@ -47,8 +47,14 @@ extern "C" {
 #define ESTL_FLAG_NORMALIZE 0x1u   /* multiply the result by 1/n                               */
 #define ESTL_FLAG_AUTOTUNE 0x2u    /* pick the plan by timing candidates on the device (cached) */
 
+/* convolution options (estl_conv_create) */
+#define ESTL_CONV_CORRELATE 0x10u        /* multiply by conj(H): correlation, Convolution.h:125-131 */
+#define ESTL_CONV_NORMALIZE_KERNEL 0x20u /* divide by the kernel sum K^[0], Convolution.h:141 */
+#define ESTL_CONV_CENTER 0x40u           /* rotate the filter by (K+1)/2, Convolution.h:113-120 */
+
 typedef struct estl_plan_s *estl_plan_t;
 typedef struct estl_ubatch_s *estl_ubatch_t;
+typedef struct estl_conv_s *estl_conv_t;
 
 /* Message of the calling thread's last failure ("" if none). */
 const char *estl_last_error(void);
@@ -143,6 +149,26 @@ int estl_ubatch_execute_graph(estl_ubatch_t ub, const void *d_in, void *d_out, v
 int estl_ubatch_describe(estl_ubatch_t ub, char *buf, int len);
 
 int estl_ubatch_destroy(estl_ubatch_t ub);
+
+/* ---------------------------------------------------------------- convolution
+ * y[r][i] = sum_t h[t] x[r][(i - t) mod n] for `rows` real rows of length n (a power of
+ * two) and a real filter of any length k (longer than n: folds modulo n), computed as
+ * transform, multiply, inverse -- the pipeline of Convolution2D_FFT::convolve
+ * (Convolution.h:95-148), batched over 1-D rows and fused on the device.  Two real rows
+ * share one complex transform.  Flags: ESTL_CONV_* above. */
+int estl_conv_create(estl_conv_t *conv, long n, long rows, int precision, unsigned flags);
+
+/* Fold (and, with ESTL_CONV_CENTER, rotate) the k-value device filter d_h and keep its
+ * spectrum resident on the device; later executes reuse it. */
+int estl_conv_set_filter(estl_conv_t conv, const float *d_h, long k, void *stream);
+
+/* y = x (*) h for all rows (device buffers of rows * n floats; asynchronous). */
+int estl_conv_execute(estl_conv_t conv, const float *d_x, float *d_y, void *stream);
+
+/* Device bytes the object holds (scratch, spectrum). */
+long long estl_conv_workspace_bytes(estl_conv_t conv);
+
+int estl_conv_destroy(estl_conv_t conv);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,119 @@
+"""GPU parity of the batched circular convolution (estl_conv_*) against the oracle's
+direct O(n k) long-double sum (oracle.conv_circular) and the numpy golden fixtures.
+
+Tolerance: relative L2 per row <= 2e-6 * log2(n) (a forward and an inverse FP32
+transform plus a complex multiply between them).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import estl_amd
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def tol(n):
+    return 2e-6 * max(1, int(math.log2(n)))
+
+
+def conv_gpu(torch, gpu, x, h, flags=0):
+    rows, n = x.shape
+    c = estl_amd.Convolution(n, rows, flags)
+    dx = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(gpu)
+    dh = torch.from_numpy(np.ascontiguousarray(h, dtype=np.float32)).to(gpu)
+    dy = torch.empty_like(dx)
+    c.set_filter(dh)
+    c.execute(dx, dy)
+    torch.cuda.synchronize()
+    return dy.cpu().numpy().astype(np.float64)
+
+
+def folded(h, n):
+    hp = np.zeros(n)
+    np.add.at(hp, np.arange(len(h)) % n, h)
+    return hp
+
+
+def check_rows(y, ref, n):
+    for r in range(ref.shape[0]):
+        assert oracle.rel_l2(y[r], ref[r]) <= tol(n), (r, oracle.rel_l2(y[r], ref[r]))
+
+
+@pytest.mark.parametrize("n,rows,k", [
+    (16, 1, 3), (64, 3, 16), (256, 4, 100), (1024, 5, 16), (4096, 2, 300), (8192, 3, 64),
+    (2 ** 14, 3, 512), (2 ** 16, 2, 1000), (2 ** 17, 1, 64), (32, 2, 40), (2 ** 14, 2, 2 ** 14 + 77),
+])
+def test_conv_matches_direct_sum(gpu, n, rows, k):
+    import torch
+    x = oracle.fill_uniform(rows * n, 4).astype(np.float64).reshape(rows, n)
+    h = oracle.fill_uniform(k, 4, offset=rows * n).astype(np.float64)
+    y = conv_gpu(torch, gpu, x.astype(np.float32), h.astype(np.float32))
+    ref = oracle.conv_circular(x.astype(np.float32).astype(np.float64), h.astype(np.float32).astype(np.float64))
+    check_rows(y, ref, n)
+
+
+def test_conv_golden_fixtures(gpu, golden):
+    import torch
+    for n, k in [(16, 3), (64, 16), (256, 100), (1024, 16), (32, 40)]:
+        x, h, yref = golden[f"conv_x_{n}_{k}"], golden[f"conv_h_{n}_{k}"], golden[f"conv_y_{n}_{k}"]
+        y = conv_gpu(torch, gpu, x.astype(np.float32), h.astype(np.float32))
+        check_rows(y, yref, n)
+
+
+@pytest.mark.parametrize("n", [256, 4096, 2 ** 15])
+def test_correlation_center_and_normalize(gpu, n):
+    import torch
+    rows, k = 3, 33
+    x = oracle.fill_uniform(rows * n, 8).astype(np.float32).reshape(rows, n)
+    h = (oracle.fill_uniform(k, 9).astype(np.float32) + 1.5).astype(np.float32)  # nonzero sum
+    x64, h64 = x.astype(np.float64), h.astype(np.float64)
+    hp = folded(h64, n)
+    # correlation: y[i] = sum_j hp[j] x[i + j] = convolution with hp reversed circularly
+    g = np.roll(hp[::-1], 1)
+    y = conv_gpu(torch, gpu, x, h, estl_amd.CONV_CORRELATE)
+    check_rows(y, oracle.conv_circular(x64, g), n)
+    # centre: the filter rotated so its index (k+1)/2 lands on 0 (Convolution.h:113-120)
+    s = (k + 1) // 2
+    y = conv_gpu(torch, gpu, x, h, estl_amd.CONV_CENTER)
+    check_rows(y, oracle.conv_circular(x64, np.roll(hp, -s)), n)
+    # normalise by the kernel sum (Convolution.h:141)
+    y = conv_gpu(torch, gpu, x, h, estl_amd.CONV_NORMALIZE_KERNEL)
+    check_rows(y, oracle.conv_circular(x64, hp / hp.sum()), n)
+
+
+def test_identity_and_delay_closed_forms(gpu):
+    import torch
+    n, rows = 2 ** 14, 5
+    x = oracle.fill_uniform(rows * n, 10).astype(np.float32).reshape(rows, n)
+    y = conv_gpu(torch, gpu, x, np.array([1.0], np.float32))
+    check_rows(y, x.astype(np.float64), n)
+    d = np.zeros(8, np.float32)
+    d[7] = 1.0
+    y = conv_gpu(torch, gpu, x, d)
+    check_rows(y, np.roll(x.astype(np.float64), 7, axis=1), n)
+
+
+def test_filter_reuse_and_odd_rows(gpu):
+    import torch
+    n, rows, k = 4096, 7, 50
+    c = estl_amd.Convolution(n, rows)
+    h = torch.from_numpy(oracle.fill_uniform(k, 12)).to(gpu)
+    c.set_filter(h)
+    ys = []
+    for seed in (1, 2):
+        x = torch.from_numpy(oracle.fill_uniform(rows * n, seed)).to(gpu)
+        y = torch.empty_like(x)
+        c.execute(x, y)
+        torch.cuda.synchronize()
+        ref = oracle.conv_circular(x.cpu().numpy().astype(np.float64).reshape(rows, n),
+                                   h.cpu().numpy().astype(np.float64))
+        check_rows(y.cpu().numpy().reshape(rows, n).astype(np.float64), ref, n)
+        ys.append(y)
+    # a second execute with the same inputs is bit-identical
+    y2 = torch.empty_like(ys[-1])
+    c.execute(x, y2)
+    torch.cuda.synchronize()
+    assert torch.equal(y2, ys[-1])
