@@ -87,6 +87,19 @@ def lib():
     _sig(L, "estl_conv_execute", ci, [vp, vp, vp, vp])
     _sig(L, "estl_conv_workspace_bytes", ll, [vp])
     _sig(L, "estl_conv_destroy", ci, [vp])
+    llp = ctypes.POINTER(ll)
+    _sig(L, "estl_server_create", ci, [ctypes.POINTER(vp), ll, ci])
+    _sig(L, "estl_server_submit_fft", ci, [vp, cl, cl, ci, vp, vp, llp])
+    _sig(L, "estl_server_submit_conv", ci, [vp, cl, cl, vp, vp, cl, cu, vp, llp])
+    _sig(L, "estl_server_poll", ci, [vp, ll, ctypes.POINTER(ci), ctypes.POINTER(vp),
+                                     ctypes.POINTER(ctypes.c_double)])
+    _sig(L, "estl_server_wait", ci, [vp, ll, ctypes.c_double])
+    _sig(L, "estl_server_copy_output", ci, [vp, ll, vp, ll, vp])
+    _sig(L, "estl_server_release", ci, [vp, ll])
+    _sig(L, "estl_server_drain", ci, [vp, ctypes.c_double])
+    _sig(L, "estl_server_stats", ci, [vp, ctypes.c_char_p, ci])
+    _sig(L, "estl_server_reset_stats", ci, [vp])
+    _sig(L, "estl_server_destroy", ci, [vp])
     _lib = L
     return L
 
@@ -236,6 +249,87 @@ class Convolution:
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
             lib().estl_conv_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
+class Server:
+    """Request loop over the FFT and convolution pipelines (C ABI estl_server_*).
+
+    Requests are admitted in arrival order against an HBM budget (one arena), run
+    concurrently on ``streams`` streams, and each result stays in the arena until
+    ``release``; ``output(id, ...)`` copies it into a new tensor.
+    """
+
+    def __init__(self, budget_bytes: int, streams: int = 4):
+        self._h = ctypes.c_void_p()
+        h = ctypes.c_void_p()
+        check(lib().estl_server_create(ctypes.byref(h), int(budget_bytes), int(streams)))
+        self._h = h
+
+    @staticmethod
+    def _ptr(t) -> int:
+        return t if isinstance(t, int) else t.data_ptr()
+
+    def submit_fft(self, n: int, batch: int, d_in, direction: int = FORWARD, stream=None) -> int:
+        if not isinstance(d_in, int) and d_in.numel() * d_in.element_size() < 8 * n * batch:
+            raise ValueError("input smaller than batch * n complex values")
+        rid = ctypes.c_longlong()
+        check(lib().estl_server_submit_fft(self._h, n, batch, direction, self._ptr(d_in), _stream_ptr(stream),
+                                           ctypes.byref(rid)))
+        return rid.value
+
+    def submit_conv(self, n: int, rows: int, d_x, d_h, k: int | None = None, flags: int = 0, stream=None) -> int:
+        if not isinstance(d_x, int) and d_x.numel() < n * rows:
+            raise ValueError("input smaller than rows * n")
+        k = k if k is not None else d_h.numel()
+        rid = ctypes.c_longlong()
+        check(lib().estl_server_submit_conv(self._h, n, rows, self._ptr(d_x), self._ptr(d_h), k, flags,
+                                            _stream_ptr(stream), ctypes.byref(rid)))
+        return rid.value
+
+    def poll(self, rid: int):
+        """(state, device address of the output or 0, latency in s or -1)."""
+        st, out, lat = ctypes.c_int(), ctypes.c_void_p(), ctypes.c_double()
+        check(lib().estl_server_poll(self._h, rid, ctypes.byref(st), ctypes.byref(out), ctypes.byref(lat)))
+        return st.value, out.value or 0, lat.value
+
+    def wait(self, rid: int, timeout: float = 60.0) -> None:
+        check(lib().estl_server_wait(self._h, rid, timeout))
+
+    def output(self, rid: int, shape, dtype, stream=None):
+        """A copy of the DONE request's output as a new torch tensor (on ``stream``)."""
+        import torch
+        t = torch.empty(*shape, dtype=dtype, device="cuda")
+        check(lib().estl_server_copy_output(self._h, rid, t.data_ptr(), t.numel() * t.element_size(),
+                                            _stream_ptr(stream)))
+        return t
+
+    def release(self, rid: int) -> None:
+        check(lib().estl_server_release(self._h, rid))
+
+    def drain(self, timeout: float = 120.0) -> None:
+        check(lib().estl_server_drain(self._h, timeout))
+
+    def stats(self) -> dict:
+        buf = ctypes.create_string_buffer(2048)
+        check(lib().estl_server_stats(self._h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def reset_stats(self) -> None:
+        check(lib().estl_server_reset_stats(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().estl_server_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

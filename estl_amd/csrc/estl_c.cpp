@@ -70,6 +70,20 @@ static TwiddleCache &twcache() {
     return *c;
 }
 
+// Tables are generated on a private stream and waited for on that stream only, so a
+// size seen for the first time does not stall work already running on the device.
+static hipStream_t table_stream(int device) {
+    static std::mutex mu;
+    static std::map<int, hipStream_t> streams;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = streams.find(device);
+    if (it != streams.end()) return it->second;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    streams[device] = s;
+    return s;
+}
+
 // Table of BlockGeom<log2n>: W_{N_q}^{g s} at [(s-1) A_q + g] per pass q >= 1.
 int block_table(int device, int log2n, const float2 **out) {
     auto &c = twcache();
@@ -83,6 +97,7 @@ int block_table(int device, int log2n, const float2 **out) {
     long count = rows_twiddle_count(log2n);
     if (count < 0) return set_error(ESTL_ERR_UNSUPPORTED, "no block kernel for log2n=%d", log2n);
     float2 *d = nullptr;
+    hipStream_t ts = table_stream(device);
     HIP_TRY(hipMalloc(&d, sizeof(float2) * (count > 0 ? count : 1)));
     long off = 0;
     for (int q = 1; q < rows_pass_count(log2n); ++q) {
@@ -90,10 +105,10 @@ int block_table(int device, int log2n, const float2 **out) {
         int B;
         rows_pass_params(log2n, q, &A, &B);
         long cnt = (long)(B - 1) * A;
-        HIP_TRY(launch_twiddle(d + off, cnt, 0, A, 0, A * B, nullptr));
+        HIP_TRY(launch_twiddle(d + off, cnt, 0, A, 0, A * B, ts));
         off += cnt;
     }
-    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipStreamSynchronize(ts));
     c.tables[key] = d;
     *out = d;
     return ESTL_OK;
@@ -110,9 +125,10 @@ int linear_table(int device, long count, long mult, long nq, const float2 **out)
         return ESTL_OK;
     }
     float2 *d = nullptr;
+    hipStream_t ts = table_stream(device);
     HIP_TRY(hipMalloc(&d, sizeof(float2) * count));
-    HIP_TRY(launch_twiddle(d, count, 1, 1, mult, nq, nullptr));
-    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(launch_twiddle(d, count, 1, 1, mult, nq, ts));
+    HIP_TRY(hipStreamSynchronize(ts));
     c.tables[key] = d;
     *out = d;
     return ESTL_OK;
@@ -229,6 +245,12 @@ static int ensure_scratch(Plan *p, int which) {
 }
 
 int execute_plan(Plan *p, const float2 *in, float2 *out, hipStream_t st) {
+    return execute_plan_ws(p, in, out, nullptr, nullptr, st);
+}
+
+// Same, with caller-provided workspace (each plan->scratch_bytes; NULL = the plan's own,
+// allocated on first use).  ws1 is needed only for in-place odd-length chains.
+int execute_plan_ws(Plan *p, const float2 *in, float2 *out, float2 *ws0, float2 *ws1, hipStream_t st) {
     if (!in || !out) return set_error(ESTL_ERR_INVALID, "null buffer");
     switch (p->kind) {
         case Plan::COPY: {
@@ -241,16 +263,23 @@ int execute_plan(Plan *p, const float2 *in, float2 *out, hipStream_t st) {
             return ESTL_OK;
         case Plan::PASSES: {
             const int P = (int)p->steps.size();
-            int rc = ensure_scratch(p, 0);
-            if (rc) return rc;
+            int rc = ESTL_OK;
+            if (!ws0) {
+                rc = ensure_scratch(p, 0);
+                if (rc) return rc;
+                ws0 = p->scratch[0];
+            }
             // destinations backwards from out, alternating with scratch 0; the first pass
             // must not write its own input, so an in-place odd-length chain uses scratch 1
             std::vector<float2 *> dst(P);
-            for (int q = P - 1, k = 0; q >= 0; --q, ++k) dst[q] = (k & 1) ? p->scratch[0] : out;
+            for (int q = P - 1, k = 0; q >= 0; --q, ++k) dst[q] = (k & 1) ? ws0 : out;
             if (dst[0] == in) {
-                rc = ensure_scratch(p, 1);
-                if (rc) return rc;
-                dst[0] = p->scratch[1];
+                if (!ws1) {
+                    rc = ensure_scratch(p, 1);
+                    if (rc) return rc;
+                    ws1 = p->scratch[1];
+                }
+                dst[0] = ws1;
             }
             const float2 *src = in;
             for (int q = 0; q < P; ++q) {

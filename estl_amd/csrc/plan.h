@@ -42,10 +42,33 @@ std::vector<int> default_pass_split(int log2n);
 int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned flags, const std::vector<int> *split,
                const std::vector<int> *tiles = nullptr);
 int execute_plan(Plan *p, const float2 *in, float2 *out, hipStream_t st);
+int execute_plan_ws(Plan *p, const float2 *in, float2 *out, float2 *ws0, float2 *ws1, hipStream_t st);
 void destroy_plan(Plan *p);
 std::string describe_plan(const Plan *p);
 int plan_launches(const Plan *p);
 int tune_plan(long n, long batch, int dir, bool *rows, std::vector<int> *split, std::vector<int> *tiles,
               std::string *report);
+
+// ---- batched circular convolution: immutable shape + per-execution buffers
+struct ConvShape {
+    int log2n = 0;
+    long n = 0, rows = 0, pairs = 0;
+    unsigned flags = 0;
+    bool corr = false, fused = false;
+    float scale = 1.0f;
+    const float2 *tw_rows = nullptr;
+    Plan spec_plan;                  // forward FFT of the folded filter (batch 1)
+    std::vector<PassStep> fwd, inv;  // multi-pass pipeline (n > 2^13)
+    size_t scratch_bytes = 0;        // bytes of each of the two scratch buffers
+    size_t spec_bytes = 0;           // bytes of the spectrum (and of the folded filter)
+};
+struct ConvBuffers {
+    float2 *spec = nullptr, *hp = nullptr, *dc = nullptr;
+    float2 *scratch[2] = {nullptr, nullptr};
+};
+int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags);
+size_t conv_footprint(const ConvShape &c);  // bytes of ConvBuffers
+int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k, hipStream_t st);
+int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y, hipStream_t st);
 
 }  // namespace estl

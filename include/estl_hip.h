@@ -55,6 +55,7 @@ extern "C" {
 typedef struct estl_plan_s *estl_plan_t;
 typedef struct estl_ubatch_s *estl_ubatch_t;
 typedef struct estl_conv_s *estl_conv_t;
+typedef struct estl_server_s *estl_server_t;
 
 /* Message of the calling thread's last failure ("" if none). */
 const char *estl_last_error(void);
@@ -169,6 +170,58 @@ int estl_conv_execute(estl_conv_t conv, const float *d_x, float *d_y, void *stre
 long long estl_conv_workspace_bytes(estl_conv_t conv);
 
 int estl_conv_destroy(estl_conv_t conv);
+
+/* ---------------------------------------------------------------- serving loop
+ * A request loop in front of the FFT and convolution pipelines.  The reference has no
+ * counterpart: its FFT objects are built and run by the caller one at a time
+ * (tests/Test.cpp:53-103) and device memory is only checked by a message
+ * (FFT_OpenCL_Contiguous.h:49).  Requests are admitted in arrival order by HBM
+ * footprint against `hbm_budget_bytes` (one arena allocated at create), launched on a
+ * pool of `streams` default-priority streams while earlier ones still run, and their
+ * workspace returns to the arena as soon as they complete.  A request whose footprint
+ * exceeds the whole budget fails with ESTL_ERR_NOMEM; a full queue returns
+ * ESTL_ERR_BUSY.  The server's streams wait for the work queued on `input_stream` at
+ * submit (the producer of the inputs); input buffers must stay valid until the request
+ * is DONE.  Results are bit-identical to the same request served alone. */
+#define ESTL_REQ_QUEUED 0
+#define ESTL_REQ_RUNNING 1
+#define ESTL_REQ_DONE 2
+#define ESTL_REQ_FAILED 3
+
+int estl_server_create(estl_server_t *srv, long long hbm_budget_bytes, int streams);
+
+/* Batched complex FFT of `batch` interleaved rows of n (FFT_OpenCL_Contiguous.h:128-161
+ * semantics); the output (batch * n complex values) lives in the arena until release. */
+int estl_server_submit_fft(estl_server_t srv, long n, long batch, int direction, const void *d_in,
+                           void *input_stream, long long *request_id);
+
+/* Circular convolution of `rows` real rows of length n with the k-value filter d_h
+ * (estl_conv_* semantics, Convolution.h:95-148); output rows * n floats in the arena. */
+int estl_server_submit_conv(estl_server_t srv, long n, long rows, const float *d_x, const float *d_h, long k,
+                            unsigned conv_flags, void *input_stream, long long *request_id);
+
+/* State (ESTL_REQ_*), the output's device address once DONE, latency submit->done. */
+int estl_server_poll(estl_server_t srv, long long request_id, int *state, void **d_out, double *latency_s);
+
+/* Block until the request is DONE (ESTL_OK), FAILED (its error) or the timeout passes
+ * (ESTL_ERR_BUSY). */
+int estl_server_wait(estl_server_t srv, long long request_id, double timeout_s);
+
+/* Copy `bytes` of a DONE request's output to d_dst on `stream` (asynchronous). */
+int estl_server_copy_output(estl_server_t srv, long long request_id, void *d_dst, long long bytes, void *stream);
+
+/* Return the request's output to the arena (after DONE or FAILED). */
+int estl_server_release(estl_server_t srv, long long request_id);
+
+/* Wait until nothing is queued or running. */
+int estl_server_drain(estl_server_t srv, double timeout_s);
+
+/* JSON statistics (counts, p50/p99/mean/max latency in ms, throughput, arena use);
+ * returns the full length. */
+int estl_server_stats(estl_server_t srv, char *json, int len);
+int estl_server_reset_stats(estl_server_t srv);
+
+int estl_server_destroy(estl_server_t srv);
 
 #ifdef __cplusplus
 }

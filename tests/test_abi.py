@@ -58,3 +58,17 @@ def test_max_batch_budget_is_64bit():
     b = estl_amd.lib().estl_max_batch(4096, estl_amd.FP32, 288 * 10 ** 9)
     assert b == 288 * 10 ** 9 // (16 * 4096)
     assert estl_amd.lib().estl_max_batch(3, estl_amd.FP32, 10 ** 9) == estl_amd.ERR_INVALID
+
+
+def test_server_rejects_bad_arguments_without_a_device():
+    L = estl_amd.lib()
+    h = ctypes.c_void_p()
+    assert L.estl_server_create(None, 1 << 20, 1) == estl_amd.ERR_INVALID
+    assert L.estl_server_create(ctypes.byref(h), 0, 1) == estl_amd.ERR_INVALID
+    assert h.value is None
+    rid = ctypes.c_longlong()
+    assert L.estl_server_submit_fft(None, 64, 1, -1, 1, None, ctypes.byref(rid)) == estl_amd.ERR_INVALID
+    assert L.estl_server_submit_conv(None, 64, 1, 1, 1, 4, 0, None, ctypes.byref(rid)) == estl_amd.ERR_INVALID
+    assert L.estl_server_wait(None, 1, 0.0) == estl_amd.ERR_INVALID
+    assert L.estl_server_release(None, 1) == estl_amd.ERR_INVALID
+    assert L.estl_server_destroy(None) == estl_amd.OK

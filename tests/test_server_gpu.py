@@ -1,0 +1,188 @@
+"""GPU tests of the serving loop (estl_server_*): each request's result, served in a
+concurrent mix, is bit-identical to the same request run alone through the batched
+entry points (estl_conv_*, estl_plan_*), which are themselves checked against the
+oracle; plus admission by HBM footprint, hold-until-release and input-stream ordering.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import estl_amd
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+# (kind, n, rows, k, flags): convolution rows of n real values with a k-value filter,
+# or a batched complex FFT of `rows` rows of n (k unused)
+MIX = [
+    ("conv", 2 ** 10, 256, 16, 0),
+    ("conv", 2 ** 14, 8, 8192, 0),
+    ("fft", 4096, 16, 0, estl_amd.FORWARD),
+    ("conv", 2 ** 16, 4, 1000, estl_amd.CONV_CORRELATE),
+    ("conv", 2 ** 12, 33, 77, estl_amd.CONV_CENTER),
+    ("fft", 2 ** 18, 2, 0, estl_amd.BACKWARD),
+    ("conv", 2 ** 20, 1, 4096, 0),
+    ("conv", 2 ** 13, 64, 300, estl_amd.CONV_NORMALIZE_KERNEL),
+    ("fft", 64, 1000, 0, estl_amd.FORWARD),
+    ("conv", 2 ** 22, 1, 16, 0),
+]
+
+
+def make_inputs(torch, gpu, case, seed):
+    kind, n, rows, k, flags = case
+    if kind == "fft":
+        x = torch.empty(rows * n, dtype=torch.complex64, device=gpu)
+        estl_amd.fill_uniform(x, seed)
+        return x, None
+    x = torch.empty(rows * n, dtype=torch.float32, device=gpu)
+    estl_amd.fill_uniform(x, seed)
+    h = torch.empty(k, dtype=torch.float32, device=gpu)
+    estl_amd.fill_uniform(h, seed + 1000)
+    if flags & estl_amd.CONV_NORMALIZE_KERNEL:
+        h += 1.0  # a filter with a nonzero sum
+    return x, h
+
+
+def alone(torch, gpu, case, x, h):
+    kind, n, rows, k, flags = case
+    if kind == "fft":
+        y = torch.empty_like(x)
+        estl_amd.Plan(n, rows, flags).execute(x, y)
+    else:
+        y = torch.empty_like(x)
+        c = estl_amd.Convolution(n, rows, flags)
+        c.set_filter(h)
+        c.execute(x, y)
+    torch.cuda.synchronize()
+    return y
+
+
+def submit(srv, case, x, h):
+    kind, n, rows, k, flags = case
+    if kind == "fft":
+        return srv.submit_fft(n, rows, x, flags)
+    return srv.submit_conv(n, rows, x, h, k, flags)
+
+
+def result(srv, rid, case, torch):
+    kind, n, rows, k, flags = case
+    dtype = torch.complex64 if kind == "fft" else torch.float32
+    t = srv.output(rid, (rows * n,), dtype)
+    torch.cuda.synchronize()
+    return t
+
+
+def test_mix_equals_alone_bitwise(gpu):
+    import torch
+    inputs = [make_inputs(torch, gpu, c, 40 + i) for i, c in enumerate(MIX)]
+    refs = [alone(torch, gpu, c, x, h) for c, (x, h) in zip(MIX, inputs)]
+    srv = estl_amd.Server(2 << 30, streams=4)
+    # every request twice, interleaved, so each runs beside different neighbours
+    order = list(range(len(MIX))) * 2
+    rng = np.random.default_rng(5)
+    rng.shuffle(order)
+    ids = [(i, submit(srv, MIX[i], *inputs[i])) for i in order]
+    srv.drain(120)
+    for i, rid in ids:
+        srv.wait(rid, 60)
+        y = result(srv, rid, MIX[i], torch)
+        assert torch.equal(y, refs[i]), (MIX[i], rid)
+        srv.release(rid)
+    st = srv.stats()
+    assert st["completed"] == len(ids) and st["failed"] == 0
+    assert st["p50_ms"] > 0 and st["p99_ms"] >= st["p50_ms"]
+    assert st["arena_used_bytes"] == 0
+    assert st["peak_inflight"] >= 2
+    srv.close()
+
+
+def test_served_results_match_oracle(gpu):
+    import torch
+    srv = estl_amd.Server(256 << 20, streams=2)
+    n, rows, k = 4096, 3, 50
+    x, h = make_inputs(torch, gpu, ("conv", n, rows, k, 0), 7)
+    rid = srv.submit_conv(n, rows, x, h)
+    srv.wait(rid, 60)
+    y = result(srv, rid, ("conv", n, rows, k, 0), torch).cpu().numpy().astype(np.float64).reshape(rows, n)
+    ref = oracle.conv_circular(x.cpu().numpy().astype(np.float64).reshape(rows, n), h.cpu().numpy().astype(np.float64))
+    for r in range(rows):
+        assert oracle.rel_l2(y[r], ref[r]) <= 2e-6 * math.log2(n)
+    srv.release(rid)
+    xf, _ = make_inputs(torch, gpu, ("fft", n, 2, 0, 0), 8)
+    rid = srv.submit_fft(n, 2, xf, estl_amd.FORWARD)
+    srv.wait(rid, 60)
+    yf = result(srv, rid, ("fft", n, 2, 0, 0), torch).cpu().numpy().astype(np.complex128).reshape(2, n)
+    ref = oracle.fft64(xf.cpu().numpy().astype(np.complex128).reshape(2, n), -1)
+    for r in range(2):
+        assert oracle.rel_l2(yf[r], ref[r]) <= 1e-6 * math.log2(n)
+    srv.release(rid)
+    srv.close()
+
+
+def test_admission_over_budget_and_hold_until_release(gpu):
+    import torch
+    n, rows = 2 ** 16, 4                     # fused-pass convolution: output 1 MiB
+    x, h = make_inputs(torch, gpu, ("conv", n, rows, 64, 0), 11)
+    ref = alone(torch, gpu, ("conv", n, rows, 64, 0), x, h)
+    # budget: an output plus one request's workspace, not two outputs plus workspace
+    out_bytes = 4 * n * rows
+    ws = 2 * 8 * n + 256 + 2 * 8 * n * (rows // 2)
+    srv = estl_amd.Server(out_bytes + ws + 4096, streams=2)
+    # a request that can never fit fails with NOMEM instead of waiting
+    big = torch.empty(2 ** 22, dtype=torch.float32, device=gpu)
+    rid_big = srv.submit_conv(2 ** 22, 1, big, h)
+    with pytest.raises(estl_amd.EstlError) as e:
+        srv.wait(rid_big, 30)
+    assert e.value.code == estl_amd.ERR_NOMEM
+    srv.release(rid_big)
+    r1 = srv.submit_conv(n, rows, x, h)
+    r2 = srv.submit_conv(n, rows, x, h)
+    srv.wait(r1, 60)
+    # r1's output still holds the arena: r2 cannot be admitted until it is released
+    with pytest.raises(estl_amd.EstlError) as e:
+        srv.wait(r2, 0.5)
+    assert e.value.code == estl_amd.ERR_BUSY
+    assert srv.poll(r2)[0] == estl_amd.REQ_QUEUED
+    assert torch.equal(result(srv, r1, ("conv", n, rows, 64, 0), torch), ref)
+    srv.release(r1)
+    srv.wait(r2, 60)
+    assert torch.equal(result(srv, r2, ("conv", n, rows, 64, 0), torch), ref)
+    srv.release(r2)
+    st = srv.stats()
+    assert st["rejected"] == 1 and st["completed"] == 2
+    assert st["arena_peak_bytes"] <= st["budget_bytes"]
+    srv.close()
+
+
+def test_input_stream_ordering(gpu):
+    import torch
+    n, rows, k = 2 ** 14, 16, 100
+    x0, h = make_inputs(torch, gpu, ("conv", n, rows, k, 0), 21)
+    ref = alone(torch, gpu, ("conv", n, rows, k, 0), x0, h)
+    srv = estl_amd.Server(512 << 20, streams=2)
+    side = torch.cuda.Stream()
+    x = torch.empty_like(x0)
+    with torch.cuda.stream(side):
+        torch.cuda._sleep(2_000_000)   # the input arrives late on the producer's stream
+        x.copy_(x0)
+    rid = srv.submit_conv(n, rows, x, h, stream=side)
+    srv.wait(rid, 60)
+    assert torch.equal(result(srv, rid, ("conv", n, rows, k, 0), torch), ref)
+    srv.release(rid)
+    srv.close()
+
+
+def test_invalid_requests_fail_cleanly(gpu):
+    import torch
+    srv = estl_amd.Server(64 << 20, streams=1)
+    x = torch.zeros(1000, dtype=torch.float32, device=gpu)
+    rid = srv.submit_conv(1000, 1, x, x, 10)   # not a power of two
+    with pytest.raises(estl_amd.EstlError) as e:
+        srv.wait(rid, 30)
+    assert e.value.code == estl_amd.ERR_INVALID
+    srv.release(rid)
+    with pytest.raises(estl_amd.EstlError):
+        srv.release(rid)                         # unknown after release
+    assert srv.stats()["failed"] == 1
+    srv.close()
