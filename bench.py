@@ -130,6 +130,158 @@ def bench_uneven(torch, estl_amd, stream, reps: int = 10):
             "tone_rel_l2_max": worst, "work_items": ub.info["items"], "workgroups": ub.info["workgroups"]}
 
 
+def _drive(srv, reqs, xp, hp, arrivals, filters=None, keep=()):
+    """Submit request i at time arrivals[i] (s from start) -- with its registered filter
+    when `filters` maps (n, k) to one, else with the filter inline -- and release each
+    output as soon as the completion queue reports it; outputs of the indices in `keep`
+    are held and returned."""
+    held, where = {}, {}
+    keep = set(keep)
+    i, count = 0, len(reqs.n)
+    t0 = time.perf_counter()
+    while i < count or where:
+        now = time.perf_counter() - t0
+        while i < count and arrivals[i] <= now:
+            n, rows, k, fl = int(reqs.n[i]), int(reqs.rows[i]), int(reqs.k[i]), int(reqs.flags[i])
+            if filters is not None:
+                rid = srv.submit_conv_filter(filters[(n, k)], rows, xp, fl)
+            else:
+                rid = srv.submit_conv(n, rows, xp, hp, k, fl)
+            where[rid] = i
+            i += 1
+        wait = max(0.0, arrivals[i] - (time.perf_counter() - t0)) if i < count else 0.05
+        for rid in srv.next_done(1024, min(wait, 0.05)):
+            j = where.pop(rid)
+            if j in keep:
+                held[j] = rid
+            else:
+                srv.release(rid)
+    return time.perf_counter() - t0, held
+
+
+def bench_serve(torch, estl_amd, stream, load: float = 0.7, budget: int = 16 << 30, streams: int = 8):
+    """Served convolution stream: 1000 requests (seed 5), n = 2^U{10..22}, k = 2^U{4..13},
+    rows U{1..256} capped at 2^22 values, 25 % correlations, fp32, filters registered
+    once per (n, k) and resident.  All requests at once (capacity), then Poisson
+    arrivals at `load` x capacity: p50/p99 latency (submit -> completion observed),
+    throughput, a bitwise alone-vs-mix check; beside two plain forms on one stream -- one
+    blocking set_filter + execute per request, and one execute per transform (each real
+    row on its own, the filter set once per request) -- and the inline-filter form."""
+    import estl_amd.workloads as wl
+    reqs = wl.conv_requests()
+    count = len(reqs.n)
+    maxv = int(reqs.values().max())
+    xp = torch.empty(maxv, dtype=torch.float32, device="cuda:0")
+    hp = torch.empty(int(reqs.k.max()), dtype=torch.float32, device="cuda:0")
+    estl_amd.fill_uniform(xp, 5, stream=stream)
+    estl_amd.fill_uniform(hp, 6, stream=stream)
+    stream.synchronize()
+    alg_bytes = float(8 * reqs.values().sum())
+    srv = estl_amd.Server(budget, streams=streams)
+    filters = {}
+    for n, k in sorted(set(zip(reqs.n.tolist(), reqs.k.tolist()))):
+        filters[(n, k)] = srv.register_filter(n, hp, k)
+    for f in filters.values():
+        srv.wait(f, 60)
+    # warm the shapes once (one request per (n, rows, flags)), untimed
+    shapes = {}
+    for j in range(count):
+        shapes.setdefault((int(reqs.n[j]), int(reqs.rows[j]), int(reqs.flags[j])), j)
+    for j in shapes.values():
+        rid = srv.submit_conv_filter(filters[(int(reqs.n[j]), int(reqs.k[j]))], int(reqs.rows[j]), xp,
+                                     int(reqs.flags[j]))
+        srv.wait(rid, 60)
+        srv.release(rid)
+    srv.next_done(4096, 0.0)
+    # capacity: everything submitted at t = 0
+    srv.reset_stats()
+    wall_cap, _ = _drive(srv, reqs, xp, hp, [0.0] * count, filters)
+    cap = srv.stats()
+    capacity = count / wall_cap
+    # inline filters (spectrum computed per request), capacity only
+    srv.reset_stats()
+    wall_inline, _ = _drive(srv, reqs, xp, hp, [0.0] * count)
+    inline = srv.stats()
+    # open loop: Poisson arrivals at load x capacity
+    rate = load * capacity
+    arrivals = (reqs.gaps / rate).cumsum()
+    sample = [0, 1, 2, 500, 999]
+    srv.reset_stats()
+    wall, held = _drive(srv, reqs, xp, hp, arrivals, filters, keep=sample)
+    st = srv.stats()
+    same = True
+    for j, rid in held.items():
+        n, rows, k, fl = int(reqs.n[j]), int(reqs.rows[j]), int(reqs.k[j]), int(reqs.flags[j])
+        y = srv.output(rid, (rows * n,), torch.float32)
+        c = estl_amd.Convolution(n, rows, fl)
+        c.set_filter(hp, k)
+        z = torch.empty_like(y)
+        c.execute(xp, z)
+        torch.cuda.synchronize()
+        same = same and torch.equal(y, z)
+        srv.release(rid)
+    failed = cap["failed"] + inline["failed"] + st["failed"]
+    srv.close()
+    # plain form: one blocking call per request (objects per shape built beforehand)
+    objs = {key: estl_amd.Convolution(key[0], key[1], key[2]) for key in shapes}
+    y = torch.empty(maxv, dtype=torch.float32, device="cuda:0")
+    sp = stream.cuda_stream
+
+    def plain():
+        for j in range(count):
+            n, rows, k, fl = int(reqs.n[j]), int(reqs.rows[j]), int(reqs.k[j]), int(reqs.flags[j])
+            c = objs[(n, rows, fl)]
+            c.set_filter(hp, k, sp)
+            c.execute(xp, y, sp)
+            stream.synchronize()
+
+    plain()
+    t0 = time.perf_counter()
+    plain()
+    wall_plain = time.perf_counter() - t0
+    del objs
+    singles = {}
+    for j in range(count):
+        key = (int(reqs.n[j]), int(reqs.flags[j]))
+        if key not in singles:
+            singles[key] = estl_amd.Convolution(key[0], 1, key[1])
+    launches = 0
+
+    def per_transform():
+        nonlocal launches
+        launches = 0
+        for j in range(count):
+            n, rows, k, fl = int(reqs.n[j]), int(reqs.rows[j]), int(reqs.k[j]), int(reqs.flags[j])
+            c = singles[(n, fl)]
+            c.set_filter(hp, k, sp)
+            xb, yb = xp.data_ptr(), y.data_ptr()
+            for r in range(rows):
+                c.execute(xb + 4 * r * n, yb + 4 * r * n, sp)
+            launches += rows
+        stream.synchronize()
+
+    per_transform()
+    t0 = time.perf_counter()
+    per_transform()
+    wall_pt = time.perf_counter() - t0
+    del singles
+    return {"workload": "served convolution stream: 1000 requests (seed 5), n=2^U{10..22}, k=2^U{4..13}, "
+                        "rows U{1..256} capped at 2^22 values, 25% correlation, fp32, filters registered per "
+                        "(n, k), Poisson arrivals",
+            "budget_bytes": budget, "streams": streams, "filters": len(filters),
+            "capacity_req_s": round(capacity, 1), "capacity_gbs": round(alg_bytes / wall_cap / 1e9, 1),
+            "capacity_p50_ms": cap["p50_ms"], "capacity_p99_ms": cap["p99_ms"],
+            "inline_filter_capacity_req_s": round(count / wall_inline, 1),
+            "plain_req_s": round(count / wall_plain, 1), "served_over_plain": round(wall_plain / wall_cap, 2),
+            "per_transform_req_s": round(count / wall_pt, 1), "per_transform_executes": launches,
+            "served_over_per_transform": round(wall_pt / wall_cap, 2),
+            "offered_req_s": round(rate, 1), "load": load,
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "throughput_req_s": round(count / wall, 1), "throughput_gbs": round(alg_bytes / wall / 1e9, 1),
+            "arena_peak_bytes": st["arena_peak_bytes"], "peak_inflight": st["peak_inflight"],
+            "rejected": st["rejected"], "failed": failed, "alone_equals_mix": same}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +289,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-size batch leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served convolution stream leg")
     ap.add_argument("--percall-rows", type=int, default=BATCH)
     ap.add_argument("--graph", action="store_true",
                     help="replay the step's two launches as one HIP graph (measured: no gain over eager launches)")
@@ -275,6 +428,7 @@ def main() -> int:
     err_c1 = oracle.rel_l2(o1.cpu().numpy()[0], oracle.fft64(a1.cpu().numpy()[0]))
 
     uneven = None if args.no_uneven else bench_uneven(torch, estl_amd, stream)
+    serve = None if args.no_serve else bench_serve(torch, estl_amd, stream)
 
     cpu = None if args.no_cpu else cpu_baseline(BATCH, 3)
 
@@ -309,6 +463,7 @@ def main() -> int:
         "rel_l2_vs_fp64": {"forward_sampled_rows": err_fwd, "round_trip": rt, "n65536": err_c1,
                            "bound": 1e-6 * math.log2(N)},
         "uneven_batch": uneven,
+        "served_convolution": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

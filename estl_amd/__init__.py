@@ -91,6 +91,9 @@ def lib():
     _sig(L, "estl_server_create", ci, [ctypes.POINTER(vp), ll, ci])
     _sig(L, "estl_server_submit_fft", ci, [vp, cl, cl, ci, vp, vp, llp])
     _sig(L, "estl_server_submit_conv", ci, [vp, cl, cl, vp, vp, cl, cu, vp, llp])
+    _sig(L, "estl_server_register_filter", ci, [vp, cl, vp, cl, cu, vp, llp])
+    _sig(L, "estl_server_submit_conv_filter", ci, [vp, ll, cl, vp, cu, vp, llp])
+    _sig(L, "estl_server_next_done", ci, [vp, llp, ci, ctypes.c_double])
     _sig(L, "estl_server_poll", ci, [vp, ll, ctypes.POINTER(ci), ctypes.POINTER(vp),
                                      ctypes.POINTER(ctypes.c_double)])
     _sig(L, "estl_server_wait", ci, [vp, ll, ctypes.c_double])
@@ -295,6 +298,26 @@ class Server:
         check(lib().estl_server_submit_conv(self._h, n, rows, self._ptr(d_x), self._ptr(d_h), k, flags,
                                             _stream_ptr(stream), ctypes.byref(rid)))
         return rid.value
+
+    def register_filter(self, n: int, d_h, k: int | None = None, flags: int = 0, stream=None) -> int:
+        """Keep the spectrum of filter d_h for rows of length n resident; returns its id."""
+        k = k if k is not None else d_h.numel()
+        fid = ctypes.c_longlong()
+        check(lib().estl_server_register_filter(self._h, n, self._ptr(d_h), k, flags, _stream_ptr(stream),
+                                                ctypes.byref(fid)))
+        return fid.value
+
+    def submit_conv_filter(self, filter_id: int, rows: int, d_x, flags: int = 0, stream=None) -> int:
+        rid = ctypes.c_longlong()
+        check(lib().estl_server_submit_conv_filter(self._h, filter_id, rows, self._ptr(d_x), flags,
+                                                   _stream_ptr(stream), ctypes.byref(rid)))
+        return rid.value
+
+    def next_done(self, max_ids: int = 256, timeout: float = 1.0) -> list:
+        """Ids of requests that finished since the last call (oldest first)."""
+        buf = (ctypes.c_longlong * max_ids)()
+        c = check(lib().estl_server_next_done(self._h, buf, max_ids, timeout))
+        return list(buf[:c])
 
     def poll(self, rid: int):
         """(state, device address of the output or 0, latency in s or -1)."""

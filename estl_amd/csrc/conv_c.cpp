@@ -4,14 +4,16 @@
 // A ConvShape (immutable per n, rows, flags) picks the pipeline:
 //   n <= 2^13: k_conv_rows, one launch (pair-pack, forward FFT, spectrum multiply,
 //              inverse FFT, scale, unpack);
-//   n >  2^13: the forward global passes -- the first reading real pairs, the last
-//              multiplying by the spectrum -- then the inverse passes, the last scaling
-//              and unpacking (pass_tile.h I/O modes).
+//   n >  2^13: the forward global passes (the first reading real pairs), one middle
+//              pass fusing the last forward pass, the spectrum multiply and the first
+//              inverse pass, then the remaining inverse passes (the last scaling and
+//              unpacking): 2P - 1 launches for a P-pass split (pass_tile.h I/O modes).
 // ConvBuffers (spectrum, folded filter, two scratch buffers) are separate so a caller
 // can place them: the estl_conv_* object owns its own, the server takes them from its
 // HBM arena.  The filter spectrum is computed once per filter and stays resident.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -47,20 +49,28 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags) {
         c->scratch_bytes = 0;
         return block_table(current_device(), c->log2n, &c->tw_rows);
     }
+    // forward passes with the default split; the inverse runs the radices in the reverse
+    // order, so the last forward pass and the first inverse pass cover the same columns
+    // and fuse (with the spectrum multiply) into one middle pass (pass_tile.h, MID)
     Plan f, b;
     rc = build_plan(&f, n, c->pairs, ESTL_FORWARD, ESTL_PRECISION_FP32, 0, nullptr);
-    if (!rc) rc = build_plan(&b, n, c->pairs, ESTL_BACKWARD, ESTL_PRECISION_FP32, 0, nullptr);
     if (rc) return rc;
+    std::vector<int> rev;
+    for (auto it = f.steps.rbegin(); it != f.steps.rend(); ++it) rev.push_back(it->log2b);
+    rc = build_plan(&b, n, c->pairs, ESTL_BACKWARD, ESTL_PRECISION_FP32, 0, &rev);
+    if (rc) return rc;
+    const size_t P = f.steps.size();
+    if (P < 2 || b.steps.size() != P) return set_error(ESTL_ERR_UNSUPPORTED, "convolution of n=%ld needs >= 2 passes", n);
     c->fused = false;
-    c->fwd = f.steps;
-    c->inv = b.steps;
+    c->fwd.assign(f.steps.begin(), f.steps.end() - 1);
+    c->mid = f.steps.back();
+    c->mid.args.corr = c->corr ? 1 : 0;
+    c->inv.assign(b.steps.begin() + 1, b.steps.end());
     // the first pass reads real pairs through its direct (S >= T) load path
     PassStep &f0 = c->fwd.front();
     while ((1 << f0.log2t) > (1L << f0.args.log2S) && f0.log2t > 0) --f0.log2t;
     f0.args.io_in = 1;
     f0.args.real_rows = rows;
-    c->fwd.back().args.io_out = 1;
-    c->fwd.back().args.corr = c->corr ? 1 : 0;
     c->inv.back().args.io_out = 2;
     c->inv.back().args.real_rows = rows;
     c->scratch_bytes = sizeof(float2) * (size_t)n * (size_t)c->pairs;
@@ -108,7 +118,8 @@ int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y,
         a.spectrum = b.spec;
         if (a.io_out == 2) a.scale = c.scale;
         float2 *dst = b.scratch[cur];
-        hipError_t e = launch_pass(s.log2b, s.log2t, dir, src, dst, a, st);
+        hipError_t e = dir ? launch_pass(s.log2b, s.log2t, dir, src, dst, a, st)
+                           : launch_pass_mid(s.log2b, s.log2t, src, dst, a, st);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "conv pass launch: %s", hipGetErrorString(e));
         src = dst;
         cur ^= 1;
@@ -116,6 +127,7 @@ int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y,
     };
     for (const auto &s : c.fwd)
         if (int rc = run(s, ESTL_FORWARD)) return rc;
+    if (int rc = run(c.mid, 0)) return rc;
     for (const auto &s : c.inv)
         if (int rc = run(s, ESTL_BACKWARD)) return rc;
     return ESTL_OK;

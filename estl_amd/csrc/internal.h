@@ -20,7 +20,8 @@ struct PassArgs {
     const float2 *tw_hi;     // W_{A B}^{hi << lobits}
     // I/O modes of the convolution pipeline (0 = plain complex rows in and out):
     //   io_in  = 1: complex row r is the real pair (rin row 2r, rin row 2r + 1)
-    //   io_out = 1: multiply the (natural-order) result by spectrum[k] (conj if corr)
+    //   spectrum / corr: the convolution middle pass (launch_pass_mid) multiplies its
+    //   natural-order bins by spectrum[k] (its conjugate if corr)
     //   io_out = 2: write re / im of the scaled result to rout rows 2r, 2r + 1
     int io_in, io_out, corr;
     long real_rows;          // real rows behind the pairs (the last pair may be single)
@@ -33,6 +34,8 @@ hipError_t launch_rows(int log2n, int dir, const float2 *in, float2 *out, long b
                        hipStream_t st);
 hipError_t launch_pass_fwd(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st);
 hipError_t launch_pass_bwd(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st);
+// forward last pass + spectrum multiply + inverse first pass of a convolution
+hipError_t launch_pass_mid(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st);
 inline hipError_t launch_pass(int log2b, int log2t, int dir, const float2 *in, float2 *out, const PassArgs &a,
                               hipStream_t st) {
     return dir < 0 ? launch_pass_fwd(log2b, log2t, in, out, a, st) : launch_pass_bwd(log2b, log2t, in, out, a, st);

@@ -13,7 +13,7 @@
 
 namespace estl {
 
-template <int LOG2B, int DIR, int LOG2T>
+template <int LOG2B, int DIR, int LOG2T, bool MID>
 __global__ void __launch_bounds__(BlockGeom<LOG2B>::TPT << LOG2T)
 k_fft_pass(const float2 *__restrict__ in, float2 *__restrict__ out, PassArgs a) {
     __shared__ float2 lds[PassTile<LOG2B, LOG2T>::lds_size];
@@ -25,10 +25,10 @@ k_fft_pass(const float2 *__restrict__ in, float2 *__restrict__ out, PassArgs a) 
         const long nb = gridDim.x;
         if ((nb & 7) == 0) tile = (tile & 7) * (nb >> 3) + (tile >> 3);
     }
-    pass_tile<LOG2B, DIR, LOG2T>(in, out, a, tile, lds, false);
+    pass_tile<LOG2B, DIR, LOG2T, MID>(in, out, a, tile, lds, false);
 }
 
-template <int LOG2B, int LOG2T>
+template <int LOG2B, int LOG2T, bool MID>
 static hipError_t launch_t(const float2 *in, float2 *out, const PassArgs &a, hipStream_t st) {
     if constexpr (LOG2B + LOG2T > kMaxPassTileLog2) {
         return hipErrorInvalidValue;
@@ -36,22 +36,22 @@ static hipError_t launch_t(const float2 *in, float2 *out, const PassArgs &a, hip
         const int lcpr = a.log2n - LOG2B;
         if (lcpr < LOG2T) return hipErrorInvalidValue;
         const long blocks = a.batch << (lcpr - LOG2T);
-        hipLaunchKernelGGL((k_fft_pass<LOG2B, ESTL_PASS_DIR, LOG2T>), dim3((unsigned)blocks),
+        hipLaunchKernelGGL((k_fft_pass<LOG2B, ESTL_PASS_DIR, LOG2T, MID>), dim3((unsigned)blocks),
                            dim3(BlockGeom<LOG2B>::TPT << LOG2T), 0, st, in, out, a);
         return hipGetLastError();
     }
 }
 
-template <int LOG2B>
+template <int LOG2B, bool MID = false>
 static hipError_t launch_b(int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st) {
     switch (log2t) {
-        case 0: return launch_t<LOG2B, 0>(in, out, a, st);
-        case 1: return launch_t<LOG2B, 1>(in, out, a, st);
-        case 2: return launch_t<LOG2B, 2>(in, out, a, st);
-        case 3: return launch_t<LOG2B, 3>(in, out, a, st);
-        case 4: return launch_t<LOG2B, 4>(in, out, a, st);
-        case 5: return launch_t<LOG2B, 5>(in, out, a, st);
-        case 6: return launch_t<LOG2B, 6>(in, out, a, st);
+        case 0: return launch_t<LOG2B, 0, MID>(in, out, a, st);
+        case 1: return launch_t<LOG2B, 1, MID>(in, out, a, st);
+        case 2: return launch_t<LOG2B, 2, MID>(in, out, a, st);
+        case 3: return launch_t<LOG2B, 3, MID>(in, out, a, st);
+        case 4: return launch_t<LOG2B, 4, MID>(in, out, a, st);
+        case 5: return launch_t<LOG2B, 5, MID>(in, out, a, st);
+        case 6: return launch_t<LOG2B, 6, MID>(in, out, a, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -78,5 +78,25 @@ hipError_t ESTL_PASS_LAUNCHER(int log2b, int log2t, const float2 *in, float2 *ou
         default: return hipErrorInvalidValue;
     }
 }
+
+#if ESTL_PASS_DIR < 0
+// the convolution's fused middle pass: last forward pass, spectrum multiply, first
+// inverse pass (pass_tile.h, MID)
+hipError_t launch_pass_mid(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st) {
+    switch (log2b) {
+        case 4: return launch_b<4, true>(log2t, in, out, a, st);
+        case 5: return launch_b<5, true>(log2t, in, out, a, st);
+        case 6: return launch_b<6, true>(log2t, in, out, a, st);
+        case 7: return launch_b<7, true>(log2t, in, out, a, st);
+        case 8: return launch_b<8, true>(log2t, in, out, a, st);
+        case 9: return launch_b<9, true>(log2t, in, out, a, st);
+        case 10: return launch_b<10, true>(log2t, in, out, a, st);
+        case 11: return launch_b<11, true>(log2t, in, out, a, st);
+        case 12: return launch_b<12, true>(log2t, in, out, a, st);
+        case 13: return launch_b<13, true>(log2t, in, out, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+#endif
 
 }  // namespace estl

@@ -35,7 +35,14 @@ struct PassTile {
 // (PassTile::threads of them) must call it together.  `lds` holds PassTile::lds_size
 // float2; if `lds_busy`, a previous user of it may still be reading and a barrier
 // comes first.
-template <int LOG2B, int DIR, int LOG2T>
+//
+// MID (forward only): the last forward pass of a convolution fused with the first
+// inverse pass.  The last forward pass (S = 1) of column j stores bin j + h n/B; the
+// first pass of an inverse whose radices run in the reverse order (A = 1, S = n/B, no
+// pre-twiddle) loads exactly x_s = in[j + s n/B] for its column j.  So the tile
+// multiplies its B bins by the spectrum and runs the inverse radix-B DFT in registers,
+// saving one HBM round trip and one launch.
+template <int LOG2B, int DIR, int LOG2T, bool MID = false>
 __device__ __forceinline__ void pass_tile(const float2 *__restrict__ in, float2 *__restrict__ out, const PassArgs &a,
                                           long tile, float2 *lds, bool lds_busy) {
     using G = BlockGeom<LOG2B>;
@@ -110,13 +117,14 @@ __device__ __forceinline__ void pass_tile(const float2 *__restrict__ in, float2 
         }
         return;
     }
-    if (a.io_out == 1) {
-        // the last forward pass writes natural order: output h of column j is bin j + h n/B
+    if constexpr (MID) {
+        // the last forward pass yields natural order: output h of column j is bin j + h n/B
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const float2 hk = a.spectrum[j + ((tl + TPT * i) << lcpr)];
             v[i] = a.corr ? cmulc(v[i], hk) : cmul(v[i], hk);
         }
+        block_fft<LOG2B, -DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, true);
     }
     float2 *dst = out + (row << a.log2n) + j;
 #pragma unroll

@@ -58,7 +58,8 @@ struct ConvShape {
     float scale = 1.0f;
     const float2 *tw_rows = nullptr;
     Plan spec_plan;                  // forward FFT of the folded filter (batch 1)
-    std::vector<PassStep> fwd, inv;  // multi-pass pipeline (n > 2^13)
+    std::vector<PassStep> fwd, inv;  // multi-pass pipeline (n > 2^13): fwd, mid, inv
+    PassStep mid{};                  // last forward + multiply + first inverse pass
     size_t scratch_bytes = 0;        // bytes of each of the two scratch buffers
     size_t spec_bytes = 0;           // bytes of the spectrum (and of the folded filter)
 };

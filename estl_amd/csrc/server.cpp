@@ -13,10 +13,15 @@
 //     and run concurrently with the ones already on the device; their memory is
 //     returned the moment they complete, admitting the next;
 //   * resident state: plans, twiddle tables and convolution shapes are cached per size
-//     and stay on the device;
+//     and stay on the device; a filter can be registered once (its spectrum then lives
+//     in the arena, counted against the budget) and used by any number of requests;
 //   * the same plan and kernels run for a request whatever runs beside it, and no
 //     value crosses requests, so each result is bit-identical alone and in the mix;
-//   * latency (submit -> completion observed) and throughput are recorded per request.
+//   * latency (submit -> completion observed) and throughput are recorded per request;
+//     completed ids are delivered through a completion queue.
+// One dispatcher thread admits requests in arrival order; one launcher thread per stream
+// issues their HIP calls (outside the lock, streams in parallel); one completer thread
+// watches the requests' events.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -90,24 +95,33 @@ class Arena {
     size_t size() const { return size_; }
     size_t used() const { return used_; }
     size_t peak() const { return peak_; }
+    // bytes held for the long term (registered filter spectra): a request larger than
+    // size() - held() could never be admitted
+    void hold(size_t bytes) { held_ += round(bytes); }
+    void unhold(size_t bytes) { held_ -= round(bytes); }
+    size_t held() const { return held_; }
 
   private:
     void *base_ = nullptr;
-    size_t size_ = 0, used_ = 0, peak_ = 0;
+    static size_t round(size_t bytes) { return bytes ? (bytes + 255) & ~size_t(255) : 256; }
+    size_t size_ = 0, used_ = 0, peak_ = 0, held_ = 0;
     std::map<size_t, size_t> free_;
 };
 
 enum ReqState { QUEUED = 0, RUNNING = 1, DONE = 2, FAILED = 3 };
+enum ReqKind { KIND_FFT = 0, KIND_CONV = 1, KIND_FILTER = 2, KIND_CONV_REG = 3 };
 
 struct Request {
     long long id = 0;
-    int kind = 0;  // 0 FFT, 1 convolution
+    int kind = KIND_FFT;
     long n = 0, rows = 0, k = 0;
     int dir = ESTL_FORWARD;
     unsigned flags = 0;
     const void *in = nullptr;
     const float *h = nullptr;
-    // arena blocks: [0] output, the rest workspace
+    long long filter = 0;  // KIND_CONV_REG: the registration it uses
+    int users = 0;         // KIND_FILTER: requests queued or running on it
+    // arena blocks: [0] output (the spectrum for KIND_FILTER), the rest workspace
     std::vector<std::pair<size_t, size_t>> blocks;
     void *out = nullptr;
     int state = QUEUED, error = ESTL_OK;
@@ -116,6 +130,11 @@ struct Request {
     hipEvent_t ev = nullptr;     // recorded after the request's last launch
     hipEvent_t ev_in = nullptr;  // recorded on the submitter's stream: inputs are ready
     double alg_bytes = 0;
+    // resolved at admission (the launcher threads never touch the caches)
+    Plan *plan = nullptr;
+    ConvShape *shape = nullptr;
+    const float2 *spectrum = nullptr;
+    int stream = 0;
 };
 
 class Server {
@@ -133,6 +152,9 @@ class Server {
             streams_.push_back(s);
         }
         max_inflight_ = 2 * nstreams_;
+        work_.resize(nstreams_);
+        cv_work_ = std::vector<std::condition_variable>(nstreams_);
+        for (int i = 0; i < nstreams_; ++i) launchers_.emplace_back([this, i] { launch_loop(i); });
         dispatcher_ = std::thread([this] { dispatch_loop(); });
         completer_ = std::thread([this] { complete_loop(); });
         return ESTL_OK;
@@ -145,12 +167,19 @@ class Server {
         }
         cv_dispatch_.notify_all();
         if (dispatcher_.joinable()) dispatcher_.join();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (auto &cv : cv_work_) cv.notify_all();
+        }
+        for (auto &t : launchers_)
+            if (t.joinable()) t.join();
         if (completer_.joinable()) completer_.join();
         for (auto s : streams_) (void)hipStreamDestroy(s);
         for (auto &kv : reqs_) {
             if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
             if (kv.second.ev_in) (void)hipEventDestroy(kv.second.ev_in);
         }
+        for (auto e : events_) (void)hipEventDestroy(e);
     }
 
     int submit(Request r, hipStream_t input_stream, long long *id) {
@@ -160,20 +189,30 @@ class Server {
             ++busy_;
             return set_error(ESTL_ERR_BUSY, "request queue full (%ld)", max_queue_);
         }
+        Request *filter = nullptr;
+        if (r.kind == KIND_CONV_REG) {
+            auto it = reqs_.find(r.filter);
+            if (it == reqs_.end() || it->second.kind != KIND_FILTER)
+                return set_error(ESTL_ERR_INVALID, "%lld is not a registered filter", r.filter);
+            if (it->second.state == FAILED) return set_error(ESTL_ERR_INVALID, "filter %lld failed", r.filter);
+            filter = &it->second;
+            r.n = filter->n;
+        }
         // the server's streams wait for the work already queued on the submitter's stream
-        if (hipEventCreateWithFlags(&r.ev_in, hipEventDisableTiming) != hipSuccess)
-            return set_error(ESTL_ERR_HIP, "event creation failed");
+        int rc = get_event(&r.ev_in);
+        if (rc) return rc;
         if (hipEventRecord(r.ev_in, input_stream) != hipSuccess) {
-            (void)hipEventDestroy(r.ev_in);
+            put_event(r.ev_in);
             return set_error(ESTL_ERR_HIP, "event record on the input stream failed");
         }
+        if (filter) ++filter->users;
         r.id = ++next_id_;
         r.t_submit = Clock::now();
-        if (first_submit_ == Clock::time_point()) first_submit_ = r.t_submit;
+        if (r.kind != KIND_FILTER && first_submit_ == Clock::time_point()) first_submit_ = r.t_submit;
         *id = r.id;
         queue_.push_back(r.id);
         reqs_.emplace(r.id, std::move(r));
-        ++submitted_;
+        if (reqs_.at(*id).kind != KIND_FILTER) ++submitted_;
         cv_dispatch_.notify_one();
         return ESTL_OK;
     }
@@ -192,7 +231,7 @@ class Server {
 
     int wait(long long id, double timeout_s) {
         std::unique_lock<std::mutex> lk(mu_);
-        const auto deadline = Clock::now() + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(timeout_s));
+        const auto deadline = Clock::now() + to_duration(timeout_s);
         while (true) {
             auto it = reqs_.find(id);
             if (it == reqs_.end()) return set_error(ESTL_ERR_INVALID, "unknown request %lld", id);
@@ -201,6 +240,21 @@ class Server {
             if (cv_done_.wait_until(lk, deadline) == std::cv_status::timeout)
                 return set_error(ESTL_ERR_BUSY, "request %lld not done within %.3f s", id, timeout_s);
         }
+    }
+
+    // ids of finished (DONE or FAILED) requests, oldest first; filter registrations are
+    // not reported here
+    int next_done(long long *ids, int max, double timeout_s) {
+        std::unique_lock<std::mutex> lk(mu_);
+        const auto deadline = Clock::now() + to_duration(timeout_s);
+        while (done_q_.empty())
+            if (cv_done_.wait_until(lk, deadline) == std::cv_status::timeout) return 0;
+        int c = 0;
+        while (c < max && !done_q_.empty()) {
+            ids[c++] = done_q_.front();
+            done_q_.pop_front();
+        }
+        return c;
     }
 
     int copy_output(long long id, void *dst, long long bytes, hipStream_t st) {
@@ -220,11 +274,13 @@ class Server {
         std::lock_guard<std::mutex> lk(mu_);
         auto it = reqs_.find(id);
         if (it == reqs_.end()) return set_error(ESTL_ERR_INVALID, "unknown request %lld", id);
-        if (it->second.state == QUEUED || it->second.state == RUNNING)
-            return set_error(ESTL_ERR_BUSY, "request %lld still in progress", id);
-        for (auto &b : it->second.blocks) arena_.release(b.first, b.second);
-        if (it->second.ev) (void)hipEventDestroy(it->second.ev);
-        if (it->second.ev_in) (void)hipEventDestroy(it->second.ev_in);
+        Request &r = it->second;
+        if (r.state == QUEUED || r.state == RUNNING) return set_error(ESTL_ERR_BUSY, "request %lld still in progress", id);
+        if (r.users) return set_error(ESTL_ERR_BUSY, "filter %lld is used by %d requests", id, r.users);
+        if (r.kind == KIND_FILTER && r.state == DONE) arena_.unhold(r.blocks[0].second);
+        for (auto &b : r.blocks) arena_.release(b.first, b.second);
+        put_event(r.ev);
+        put_event(r.ev_in);
         reqs_.erase(it);
         cv_dispatch_.notify_one();  // memory came back
         return ESTL_OK;
@@ -232,8 +288,8 @@ class Server {
 
     int drain(double timeout_s) {
         std::unique_lock<std::mutex> lk(mu_);
-        const auto deadline = Clock::now() + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(timeout_s));
-        while (!queue_.empty() || !inflight_.empty())
+        const auto deadline = Clock::now() + to_duration(timeout_s);
+        while (!queue_.empty() || !inflight_.empty() || launching_)
             if (cv_done_.wait_until(lk, deadline) == std::cv_status::timeout)
                 return set_error(ESTL_ERR_BUSY, "drain timed out with %zu queued, %zu running", queue_.size(),
                                  inflight_.size());
@@ -280,6 +336,26 @@ class Server {
     }
 
   private:
+    static Clock::duration to_duration(double s) {
+        return std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(std::max(0.0, s)));
+    }
+
+    // event pool (under mu_)
+    int get_event(hipEvent_t *e) {
+        if (!events_.empty()) {
+            *e = events_.back();
+            events_.pop_back();
+            return ESTL_OK;
+        }
+        if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess)
+            return set_error(ESTL_ERR_HIP, "event creation failed");
+        return ESTL_OK;
+    }
+    void put_event(hipEvent_t &e) {
+        if (e) events_.push_back(e);
+        e = nullptr;
+    }
+
     // plan / shape caches (dispatcher thread only)
     Plan *fft_plan(long n, long batch, int dir) {
         auto key = std::make_tuple(n, batch, dir);
@@ -301,59 +377,105 @@ class Server {
         conv_shapes_[key] = std::move(c);
         return raw;
     }
+    static size_t spec_scratch(const ConvShape &c) {
+        return c.spec_plan.kind == Plan::PASSES ? c.spec_bytes : 0;
+    }
 
     // sizes of the arena blocks a request needs: output first
     bool footprint(Request &r, std::vector<size_t> &sizes, std::string &why) {
         sizes.clear();
-        if (r.kind == 0) {
-            Plan *p = fft_plan(r.n, r.rows, r.dir);
-            if (!p) {
-                why = estl_last_error();
-                return false;
+        auto fail = [&]() {
+            why = estl_last_error();
+            return false;
+        };
+        switch (r.kind) {
+            case KIND_FFT: {
+                Plan *p = fft_plan(r.n, r.rows, r.dir);
+                if (!p) return fail();
+                r.plan = p;
+                sizes.push_back(sizeof(float2) * (size_t)r.n * (size_t)r.rows);
+                if (p->kind == Plan::PASSES) sizes.push_back(p->scratch_bytes);
+                r.alg_bytes = 16.0 * r.n * r.rows;
+                return true;
             }
-            sizes.push_back(sizeof(float2) * (size_t)r.n * (size_t)r.rows);
-            if (p->kind == Plan::PASSES) sizes.push_back(p->scratch_bytes);
-            r.alg_bytes = 16.0 * r.n * r.rows;
-        } else {
-            ConvShape *c = conv_shape(r.n, r.rows, r.flags);
-            if (!c) {
-                why = estl_last_error();
-                return false;
+            case KIND_CONV:
+            case KIND_CONV_REG: {
+                ConvShape *c = conv_shape(r.n, r.rows, r.flags);
+                if (!c) return fail();
+                r.shape = c;
+                sizes.push_back(sizeof(float) * (size_t)r.n * (size_t)r.rows);
+                if (c->scratch_bytes) {
+                    sizes.push_back(c->scratch_bytes);
+                    sizes.push_back(c->scratch_bytes);
+                }
+                if (r.kind == KIND_CONV) {
+                    sizes.push_back(c->spec_bytes);
+                    sizes.push_back(c->spec_bytes);
+                    sizes.push_back(256);
+                    if (!c->scratch_bytes && spec_scratch(*c)) sizes.push_back(spec_scratch(*c));
+                }
+                r.alg_bytes = 8.0 * r.n * r.rows + (r.kind == KIND_CONV ? 4.0 * r.k : 0.0);
+                return true;
             }
-            sizes.push_back(sizeof(float) * (size_t)r.n * (size_t)r.rows);
-            sizes.push_back(c->spec_bytes);
-            sizes.push_back(c->spec_bytes);
-            sizes.push_back(256);
-            size_t sb = c->scratch_bytes ? c->scratch_bytes : (c->spec_plan.kind == Plan::PASSES ? c->spec_bytes : 0);
-            if (sb) {
-                sizes.push_back(sb);
-                sizes.push_back(sb);
+            case KIND_FILTER: {
+                ConvShape *c = conv_shape(r.n, 1, r.flags);
+                if (!c) return fail();
+                r.shape = c;
+                sizes.push_back(c->spec_bytes);
+                sizes.push_back(c->spec_bytes);
+                sizes.push_back(256);
+                if (spec_scratch(*c)) sizes.push_back(spec_scratch(*c));
+                r.alg_bytes = 0;
+                return true;
             }
-            r.alg_bytes = 8.0 * r.n * r.rows + 4.0 * r.k;
         }
-        return true;
+        why = "unknown request kind";
+        return false;
     }
 
+    // launches of one admitted request on stream st (called without the lock)
     int launch(Request &r, hipStream_t st) {
         auto ptr = [&](size_t i) -> float2 * { return reinterpret_cast<float2 *>(arena_.base() + r.blocks[i].first); };
-        r.out = arena_.base() + r.blocks[0].first;
-        if (r.kind == 0) {
-            Plan *p = fft_plan(r.n, r.rows, r.dir);
+        if (r.kind == KIND_FFT) {
+            Plan *p = r.plan;
             float2 *ws0 = r.blocks.size() > 1 ? ptr(1) : nullptr;
             return execute_plan_ws(p, static_cast<const float2 *>(r.in), static_cast<float2 *>(r.out), ws0, nullptr, st);
         }
-        ConvShape *c = conv_shape(r.n, r.rows, r.flags);
-        ConvBuffers b;
-        b.spec = ptr(1);
-        b.hp = ptr(2);
-        b.dc = ptr(3);
-        if (r.blocks.size() > 5) {
-            b.scratch[0] = ptr(4);
-            b.scratch[1] = ptr(5);
+        if (r.kind == KIND_FILTER) {
+            ConvShape *c = r.shape;
+            ConvBuffers b;
+            b.spec = ptr(0);
+            b.hp = ptr(1);
+            b.dc = ptr(2);
+            if (r.blocks.size() > 3) b.scratch[0] = ptr(3);
+            return conv_filter(*c, b, r.h, r.k, st);
         }
-        int rc = conv_filter(*c, b, r.h, r.k, st);
-        if (!rc) rc = conv_run(*c, b, static_cast<const float *>(r.in), static_cast<float *>(r.out), st);
-        return rc;
+        ConvShape *c = r.shape;
+        ConvBuffers b;
+        size_t next = 1;
+        if (c->scratch_bytes) {
+            b.scratch[0] = ptr(next++);
+            b.scratch[1] = ptr(next++);
+        }
+        if (r.kind == KIND_CONV) {
+            b.spec = ptr(next++);
+            b.hp = ptr(next++);
+            b.dc = ptr(next++);
+            if (!b.scratch[0] && next < r.blocks.size()) b.scratch[0] = ptr(next++);
+            int rc = conv_filter(*c, b, r.h, r.k, st);
+            if (rc) return rc;
+        } else {
+            b.spec = const_cast<float2 *>(r.spectrum);
+        }
+        return conv_run(*c, b, static_cast<const float *>(r.in), static_cast<float *>(r.out), st);
+    }
+
+    void finish_locked(Request &r) {
+        if (r.kind == KIND_CONV_REG) {
+            auto f = reqs_.find(r.filter);
+            if (f != reqs_.end()) --f->second.users;
+        }
+        if (r.kind != KIND_FILTER) done_q_.push_back(r.id);
     }
 
     void fail_locked(Request &r, int code, const std::string &msg) {
@@ -362,17 +484,33 @@ class Server {
         r.msg = msg;
         for (auto &b : r.blocks) arena_.release(b.first, b.second);
         r.blocks.clear();
-        ++failed_;
+        r.out = nullptr;
+        if (r.kind != KIND_FILTER) ++failed_;
+        finish_locked(r);
         cv_done_.notify_all();
     }
 
+    // Admission, in arrival order: footprint, arena blocks, a stream; the launch itself
+    // is handed to that stream's launcher thread.
     void dispatch_loop() {
         (void)hipSetDevice(device_);
         std::unique_lock<std::mutex> lk(mu_);
         while (true) {
-            cv_dispatch_.wait(lk, [&] { return stop_ || (!queue_.empty() && inflight_.size() < (size_t)max_inflight_); });
+            cv_dispatch_.wait(lk, [&] {
+                return stop_ || (!queue_.empty() && inflight_.size() + launching_ < (size_t)max_inflight_);
+            });
             if (stop_) return;
             Request &r = reqs_.at(queue_.front());
+            const Request *filter = nullptr;
+            if (r.kind == KIND_CONV_REG) {
+                auto f = reqs_.find(r.filter);  // held by r's use count
+                filter = &f->second;
+                if (filter->state == FAILED) {
+                    queue_.pop_front();
+                    fail_locked(r, ESTL_ERR_INVALID, "its filter registration failed");
+                    continue;
+                }
+            }
             std::vector<size_t> sizes;
             std::string why;
             if (!footprint(r, sizes, why)) {
@@ -382,11 +520,11 @@ class Server {
             }
             size_t total = 0;
             for (size_t s : sizes) total += (s + 255) & ~size_t(255);
-            if (total > arena_.size()) {
+            if (total > arena_.size() - arena_.held()) {
                 queue_.pop_front();
-                ++rejected_;
-                fail_locked(r, ESTL_ERR_NOMEM, "request needs " + std::to_string(total) + " bytes, budget is " +
-                                                   std::to_string(arena_.size()));
+                if (r.kind != KIND_FILTER) ++rejected_;
+                fail_locked(r, ESTL_ERR_NOMEM, "request needs " + std::to_string(total) + " bytes, the budget leaves " +
+                                                   std::to_string(arena_.size() - arena_.held()));
                 continue;
             }
             std::vector<std::pair<size_t, size_t>> blocks;
@@ -407,17 +545,52 @@ class Server {
             }
             queue_.pop_front();
             r.blocks = std::move(blocks);
+            r.out = arena_.base() + r.blocks[0].first;
             r.state = RUNNING;
-            const int s = next_stream_++ % nstreams_;
-            int rc = ESTL_OK;
-            if (hipStreamWaitEvent(streams_[s], r.ev_in, 0) != hipSuccess)
-                rc = set_error(ESTL_ERR_HIP, "stream wait on the input event failed");
-            if (!rc) rc = launch(r, streams_[s]);
-            if (!rc && hipEventCreateWithFlags(&r.ev, hipEventDisableTiming) != hipSuccess)
-                rc = set_error(ESTL_ERR_HIP, "event creation failed");
-            if (!rc && hipEventRecord(r.ev, streams_[s]) != hipSuccess) rc = set_error(ESTL_ERR_HIP, "event record failed");
+            if (r.kind == KIND_FILTER) arena_.hold(r.blocks[0].second);
+            // a request on a filter still in flight goes to the filter's stream (stream
+            // order makes the spectrum ready); others take turns
+            if (filter) {
+                r.spectrum = static_cast<const float2 *>(filter->out);
+                r.stream = filter->state == DONE ? next_stream_++ % nstreams_ : filter->stream;
+            } else {
+                r.stream = next_stream_++ % nstreams_;
+            }
+            const int rc = get_event(&r.ev);
             if (rc) {
+                if (r.kind == KIND_FILTER) arena_.unhold(r.blocks[0].second);
                 fail_locked(r, rc, estl_last_error());
+                continue;
+            }
+            ++launching_;
+            work_[r.stream].push_back(r.id);
+            cv_work_[r.stream].notify_one();
+        }
+    }
+
+    // Launcher of stream s: the HIP calls of each admitted request, outside the lock.
+    void launch_loop(int s) {
+        (void)hipSetDevice(device_);
+        const hipStream_t st = streams_[s];
+        std::unique_lock<std::mutex> lk(mu_);
+        while (true) {
+            cv_work_[s].wait(lk, [&] { return stop_ || !work_[s].empty(); });
+            if (work_[s].empty()) return;  // stopping, nothing left to launch
+            Request &r = reqs_.at(work_[s].front());
+            work_[s].pop_front();
+            lk.unlock();
+            int rc = ESTL_OK;
+            if (hipStreamWaitEvent(st, r.ev_in, 0) != hipSuccess)
+                rc = set_error(ESTL_ERR_HIP, "stream wait on the input event failed");
+            if (!rc) rc = launch(r, st);
+            if (!rc && hipEventRecord(r.ev, st) != hipSuccess) rc = set_error(ESTL_ERR_HIP, "event record failed");
+            const std::string err = rc ? estl_last_error() : std::string();
+            lk.lock();
+            --launching_;
+            if (rc) {
+                if (r.kind == KIND_FILTER) arena_.unhold(r.blocks[0].second);
+                fail_locked(r, rc, err);
+                cv_dispatch_.notify_one();
                 continue;
             }
             inflight_.push_back(r.id);
@@ -431,7 +604,7 @@ class Server {
         while (true) {
             {
                 std::lock_guard<std::mutex> lk(mu_);
-                if (stop_ && inflight_.empty()) return;
+                if (stop_ && inflight_.empty() && !launching_) return;
                 watch.clear();
                 for (long long id : inflight_) watch.emplace_back(id, reqs_.at(id).ev);
             }
@@ -442,7 +615,9 @@ class Server {
                 std::lock_guard<std::mutex> lk(mu_);
                 Request &r = reqs_.at(w.first);
                 inflight_.erase(std::find(inflight_.begin(), inflight_.end(), w.first));
+                put_event(r.ev_in);
                 if (q != hipSuccess) {
+                    if (r.kind == KIND_FILTER) arena_.unhold(r.blocks[0].second);
                     fail_locked(r, ESTL_ERR_HIP, std::string("device error: ") + hipGetErrorString(q));
                 } else {
                     r.t_done = Clock::now();
@@ -450,10 +625,13 @@ class Server {
                     // workspace back to the arena now; the output stays until release
                     for (size_t i = 1; i < r.blocks.size(); ++i) arena_.release(r.blocks[i].first, r.blocks[i].second);
                     r.blocks.resize(1);
-                    latencies_.push_back(std::chrono::duration<double>(r.t_done - r.t_submit).count());
-                    ++completed_;
-                    done_bytes_ += r.alg_bytes;
-                    last_done_ = r.t_done;
+                    if (r.kind != KIND_FILTER) {
+                        latencies_.push_back(std::chrono::duration<double>(r.t_done - r.t_submit).count());
+                        ++completed_;
+                        done_bytes_ += r.alg_bytes;
+                        last_done_ = r.t_done;
+                    }
+                    finish_locked(r);
                 }
                 any = true;
             }
@@ -461,7 +639,7 @@ class Server {
                 cv_done_.notify_all();
                 cv_dispatch_.notify_one();
             } else {
-                std::this_thread::sleep_for(std::chrono::microseconds(10));
+                std::this_thread::sleep_for(std::chrono::microseconds(5));
             }
         }
     }
@@ -470,15 +648,20 @@ class Server {
     long max_queue_ = 1 << 16;
     Arena arena_;
     std::vector<hipStream_t> streams_;
+    std::vector<hipEvent_t> events_;
     std::mutex mu_;
     std::condition_variable cv_dispatch_, cv_done_;
-    std::deque<long long> queue_;
+    std::deque<long long> queue_, done_q_;
     std::vector<long long> inflight_;
     std::map<long long, Request> reqs_;
     std::map<std::tuple<long, long, int>, std::unique_ptr<Plan>> fft_plans_;
     std::map<std::tuple<long, long, unsigned>, std::unique_ptr<ConvShape>> conv_shapes_;
     std::thread dispatcher_, completer_;
+    std::vector<std::thread> launchers_;
+    std::vector<std::deque<long long>> work_;  // admitted, to be launched on stream s
+    std::vector<std::condition_variable> cv_work_;
     bool stop_ = false;
+    size_t launching_ = 0;
     long long next_id_ = 0, submitted_ = 0, completed_ = 0, rejected_ = 0, failed_ = 0, busy_ = 0;
     size_t peak_inflight_ = 0;
     int next_stream_ = 0;
@@ -511,7 +694,7 @@ int estl_server_submit_fft(estl_server_t srv, long n, long batch, int direction,
                            void *input_stream, long long *request_id) {
     if (!srv || !request_id || !d_in) return set_error(ESTL_ERR_INVALID, "null argument");
     Request r;
-    r.kind = 0;
+    r.kind = KIND_FFT;
     r.n = n;
     r.rows = batch;
     r.dir = direction;
@@ -524,13 +707,44 @@ int estl_server_submit_conv(estl_server_t srv, long n, long rows, const float *d
     if (!srv || !request_id || !d_x || !d_h) return set_error(ESTL_ERR_INVALID, "null argument");
     if (k < 1) return set_error(ESTL_ERR_INVALID, "filter length %ld < 1", k);
     Request r;
-    r.kind = 1;
+    r.kind = KIND_CONV;
     r.n = n;
     r.rows = rows;
     r.k = k;
     r.flags = conv_flags;
     r.in = d_x;
     r.h = d_h;
+    return reinterpret_cast<Server *>(srv)->submit(std::move(r), static_cast<hipStream_t>(input_stream), request_id);
+}
+
+int estl_server_register_filter(estl_server_t srv, long n, const float *d_h, long k, unsigned conv_flags,
+                                void *input_stream, long long *filter_id) {
+    if (!srv || !filter_id || !d_h) return set_error(ESTL_ERR_INVALID, "null argument");
+    if (k < 1) return set_error(ESTL_ERR_INVALID, "filter length %ld < 1", k);
+    if (conv_flags & ESTL_CONV_CORRELATE)
+        return set_error(ESTL_ERR_INVALID, "correlation is chosen per request, not at registration");
+    Request r;
+    r.kind = KIND_FILTER;
+    r.n = n;
+    r.rows = 1;
+    r.k = k;
+    r.flags = conv_flags;
+    r.h = d_h;
+    r.in = d_h;
+    return reinterpret_cast<Server *>(srv)->submit(std::move(r), static_cast<hipStream_t>(input_stream), filter_id);
+}
+
+int estl_server_submit_conv_filter(estl_server_t srv, long long filter_id, long rows, const float *d_x,
+                                   unsigned conv_flags, void *input_stream, long long *request_id) {
+    if (!srv || !request_id || !d_x) return set_error(ESTL_ERR_INVALID, "null argument");
+    if (conv_flags & ~(unsigned)ESTL_CONV_CORRELATE)
+        return set_error(ESTL_ERR_INVALID, "only ESTL_CONV_CORRELATE is chosen per request (0x%x)", conv_flags);
+    Request r;
+    r.kind = KIND_CONV_REG;
+    r.filter = filter_id;
+    r.rows = rows;
+    r.flags = conv_flags;
+    r.in = d_x;
     return reinterpret_cast<Server *>(srv)->submit(std::move(r), static_cast<hipStream_t>(input_stream), request_id);
 }
 
@@ -542,6 +756,11 @@ int estl_server_poll(estl_server_t srv, long long request_id, int *state, void *
 int estl_server_wait(estl_server_t srv, long long request_id, double timeout_s) {
     if (!srv) return set_error(ESTL_ERR_INVALID, "handle is null");
     return reinterpret_cast<Server *>(srv)->wait(request_id, timeout_s);
+}
+
+int estl_server_next_done(estl_server_t srv, long long *request_ids, int max_ids, double timeout_s) {
+    if (!srv || !request_ids || max_ids < 1) return set_error(ESTL_ERR_INVALID, "bad argument");
+    return reinterpret_cast<Server *>(srv)->next_done(request_ids, max_ids, timeout_s);
 }
 
 int estl_server_copy_output(estl_server_t srv, long long request_id, void *d_dst, long long bytes, void *stream) {

@@ -200,12 +200,30 @@ int estl_server_submit_fft(estl_server_t srv, long n, long batch, int direction,
 int estl_server_submit_conv(estl_server_t srv, long n, long rows, const float *d_x, const float *d_h, long k,
                             unsigned conv_flags, void *input_stream, long long *request_id);
 
+/* Register a filter for later requests: its spectrum for rows of length n (with the
+ * ESTL_CONV_CENTER / ESTL_CONV_NORMALIZE_KERNEL options) is computed once and stays in
+ * the arena, counted against the budget, until estl_server_release(filter_id).  The
+ * registration is itself a request (wait on it or not; it is not reported by
+ * estl_server_next_done).  The role of the kernel transform that
+ * Convolution2D_FFT::convolve redoes on every call (Convolution.h:104-123). */
+int estl_server_register_filter(estl_server_t srv, long n, const float *d_h, long k, unsigned conv_flags,
+                                void *input_stream, long long *filter_id);
+
+/* Convolution of `rows` real rows of the filter's n with a registered filter;
+ * conv_flags may hold ESTL_CONV_CORRELATE only. */
+int estl_server_submit_conv_filter(estl_server_t srv, long long filter_id, long rows, const float *d_x,
+                                   unsigned conv_flags, void *input_stream, long long *request_id);
+
 /* State (ESTL_REQ_*), the output's device address once DONE, latency submit->done. */
 int estl_server_poll(estl_server_t srv, long long request_id, int *state, void **d_out, double *latency_s);
 
 /* Block until the request is DONE (ESTL_OK), FAILED (its error) or the timeout passes
  * (ESTL_ERR_BUSY). */
 int estl_server_wait(estl_server_t srv, long long request_id, double timeout_s);
+
+/* Completion queue: up to max_ids ids of requests that finished (DONE or FAILED), oldest
+ * first, waiting up to timeout_s for the first; returns how many (0 on timeout). */
+int estl_server_next_done(estl_server_t srv, long long *request_ids, int max_ids, double timeout_s);
 
 /* Copy `bytes` of a DONE request's output to d_dst on `stream` (asynchronous). */
 int estl_server_copy_output(estl_server_t srv, long long request_id, void *d_dst, long long bytes, void *stream);

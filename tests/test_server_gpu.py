@@ -186,3 +186,52 @@ def test_invalid_requests_fail_cleanly(gpu):
         srv.release(rid)                         # unknown after release
     assert srv.stats()["failed"] == 1
     srv.close()
+
+
+def test_registered_filters_and_completion_queue(gpu):
+    import torch
+    srv = estl_amd.Server(1 << 30, streams=3)
+    cases = [(2 ** 12, 40, 100, 0), (2 ** 16, 3, 5000, estl_amd.CONV_CORRELATE), (2 ** 10, 256, 16, 0),
+             (2 ** 18, 2, 64, estl_amd.CONV_CORRELATE)]
+    inputs, refs, fids = [], [], []
+    for i, (n, rows, k, fl) in enumerate(cases):
+        x, h = make_inputs(torch, gpu, ("conv", n, rows, k, fl), 60 + i)
+        inputs.append((x, h))
+        refs.append(alone(torch, gpu, ("conv", n, rows, k, fl), x, h))
+        fids.append(srv.register_filter(n, h, k))  # not waited on: FIFO order covers it
+    ids = {}
+    for rep in range(3):
+        for i, (n, rows, k, fl) in enumerate(cases):
+            ids[srv.submit_conv_filter(fids[i], rows, inputs[i][0], fl)] = i
+    # a filter in use cannot be released
+    with pytest.raises(estl_amd.EstlError) as e:
+        srv.release(fids[0])
+    assert e.value.code == estl_amd.ERR_BUSY
+    seen = []
+    while len(seen) < len(ids):
+        got = srv.next_done(64, 30.0)
+        assert got, "completion queue timed out"
+        seen += got
+    assert sorted(seen) == sorted(ids) and not set(fids) & set(seen)
+    for rid, i in ids.items():
+        n, rows, k, fl = cases[i]
+        assert torch.equal(result(srv, rid, ("conv", n, rows, k, fl), torch), refs[i]), cases[i]
+        srv.release(rid)
+    for f in fids:
+        srv.release(f)
+    st = srv.stats()
+    assert st["completed"] == len(ids) and st["arena_used_bytes"] == 0
+    # registration options: centre / normalise apply to the registered spectrum
+    n, rows, k = 2 ** 14, 4, 33
+    x, h = make_inputs(torch, gpu, ("conv", n, rows, k, estl_amd.CONV_NORMALIZE_KERNEL), 70)
+    for fl in (estl_amd.CONV_CENTER, estl_amd.CONV_NORMALIZE_KERNEL):
+        ref = alone(torch, gpu, ("conv", n, rows, k, fl), x, h)
+        f = srv.register_filter(n, h, k, fl)
+        rid = srv.submit_conv_filter(f, rows, x)
+        srv.wait(rid, 60)
+        assert torch.equal(result(srv, rid, ("conv", n, rows, k, fl), torch), ref)
+        srv.release(rid)
+        srv.release(f)
+    with pytest.raises(estl_amd.EstlError):
+        srv.register_filter(n, h, k, estl_amd.CONV_CORRELATE)
+    srv.close()
