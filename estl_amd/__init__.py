@@ -82,6 +82,9 @@ def lib():
     _sig(L, "estl_ubatch_execute_graph", ci, [vp, vp, vp, vp])
     _sig(L, "estl_ubatch_describe", ci, [vp, ctypes.c_char_p, ci])
     _sig(L, "estl_ubatch_destroy", ci, [vp])
+    _sig(L, "estl_pack_fp16", ci, [vp, vp, vp, cl, cl, vp])
+    _sig(L, "estl_unpack_fp16", ci, [vp, vp, vp, cl, cl, vp])
+    _sig(L, "estl_execute_fp16", ci, [vp, vp, vp, vp, vp, vp])
     _sig(L, "estl_conv_create", ci, [ctypes.POINTER(vp), cl, cl, ci, cu])
     _sig(L, "estl_conv_set_filter", ci, [vp, vp, cl, vp])
     _sig(L, "estl_conv_execute", ci, [vp, vp, vp, vp])
@@ -152,6 +155,11 @@ class Plan:
         pin = d_in if isinstance(d_in, int) else d_in.data_ptr()
         pout = d_out if isinstance(d_out, int) else d_out.data_ptr()
         check(lib().estl_execute_device(self._h, pin, pout, _stream_ptr(stream)))
+
+    def execute_fp16(self, d_in, in_scale, d_out, out_scale, stream=None) -> None:
+        """FP16-storage plan: binary16 (re, im) buffers with one FP32 scale per row."""
+        p = [t if isinstance(t, int) else t.data_ptr() for t in (d_in, in_scale, d_out, out_scale)]
+        check(lib().estl_execute_fp16(self._h, p[0], p[1], p[2], p[3], _stream_ptr(stream)))
 
     def execute_host(self, h_in, h_out) -> None:
         """Synchronous host-to-host execute (numpy arrays, complex64)."""
@@ -360,6 +368,16 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def pack_fp16(x, x16, scale, n: int, batch: int, stream=None) -> None:
+    """complex64 device tensor -> binary16 (re, im) pairs + per-row max-abs scale."""
+    check(lib().estl_pack_fp16(x.data_ptr(), x16.data_ptr(), scale.data_ptr(), n, batch, _stream_ptr(stream)))
+
+
+def unpack_fp16(x16, scale, x, n: int, batch: int, stream=None) -> None:
+    """binary16 pairs + per-row scale -> complex64 device tensor."""
+    check(lib().estl_unpack_fp16(x16.data_ptr(), scale.data_ptr(), x.data_ptr(), n, batch, _stream_ptr(stream)))
 
 
 def fill_uniform(t, seed: int, offset: int = 0, stream=None) -> None:

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <cmath>
 #include <mutex>
 #include <string>
 #include <tuple>
@@ -172,8 +173,9 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
     if (batch < 1) return set_error(ESTL_ERR_INVALID, "batch=%ld must be >= 1", batch);
     if (dir != ESTL_FORWARD && dir != ESTL_BACKWARD)
         return set_error(ESTL_ERR_INVALID, "direction=%d must be -1 or +1", dir);
-    if (precision != ESTL_PRECISION_FP32)
-        return set_error(ESTL_ERR_UNSUPPORTED, "precision %d not supported by this entry point", precision);
+    if (precision != ESTL_PRECISION_FP32 && precision != ESTL_PRECISION_FP16)
+        return set_error(ESTL_ERR_UNSUPPORTED, "precision %d not supported", precision);
+    const bool fp16 = precision == ESTL_PRECISION_FP16;
     p->device = current_device();
     p->n = n;
     p->batch = batch;
@@ -182,6 +184,12 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
     p->flags = flags;
     p->log2n = ilog2(n);
     p->scale = (flags & ESTL_FLAG_NORMALIZE) ? 1.0f / (float)n : 1.0f;
+    if (fp16) {
+        // FP16 storage: every pass stores 1/sqrt(radix) of its result (the rows kernel
+        // 1/sqrt(n)); the output scale carries sqrt(n) and any normalisation
+        p->out_factor = std::sqrt((double)n) * ((flags & ESTL_FLAG_NORMALIZE) ? 1.0 / (double)n : 1.0);
+        p->scale = (float)(1.0 / std::sqrt((double)n));
+    }
     if (n == 1) {
         p->kind = Plan::COPY;
         return ESTL_OK;
@@ -193,9 +201,9 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
     p->kind = Plan::PASSES;
     std::vector<int> bits = (split && !split->empty()) ? *split : default_pass_split(p->log2n);
     int total = 0;
+    const int max_b = fp16 ? max_pass_log2_fp16() : kMaxPassLog2;
     for (int b : bits) {
-        if (b < 4 || b > kMaxPassLog2)
-            return set_error(ESTL_ERR_UNSUPPORTED, "pass radix 2^%d outside [2^4, 2^%d]", b, kMaxPassLog2);
+        if (b < 4 || b > max_b) return set_error(ESTL_ERR_UNSUPPORTED, "pass radix 2^%d outside [2^4, 2^%d]", b, max_b);
         total += b;
     }
     if (total != p->log2n) return set_error(ESTL_ERR_INVALID, "pass split does not sum to log2n=%d", p->log2n);
@@ -209,12 +217,12 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
         st.args.A = A;
         st.args.log2n = p->log2n;
         st.args.log2S = ilog2(n / (A * B));
-        st.args.scale = (q + 1 == bits.size()) ? p->scale : 1.0f;
+        st.args.scale = fp16 ? (float)(1.0 / std::sqrt((double)B)) : (q + 1 == bits.size()) ? p->scale : 1.0f;
         st.log2t = default_pass_tile(bits[q], p->log2n - bits[q], batch * (n / B));
         st.args.xcd_remap = 0;
         if (tiles && tiles->size() == bits.size()) {
             const int lt = (*tiles)[q];
-            if (lt < 0 || lt > kMaxPassCols || lt + bits[q] > kMaxPassTileLog2 || lt > p->log2n - bits[q])
+            if (lt < 0 || lt > kMaxPassCols || lt + bits[q] > (fp16 ? 13 : kMaxPassTileLog2) || lt > p->log2n - bits[q])
                 return set_error(ESTL_ERR_UNSUPPORTED, "tile 2^%d invalid for pass radix 2^%d", lt, bits[q]);
             st.log2t = lt;
             // tiles narrower than a 128-byte line share lines with their neighbours
@@ -234,11 +242,11 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
         p->steps.push_back(st);
         A *= B;
     }
-    p->scratch_bytes = sizeof(float2) * (size_t)n * (size_t)batch;
+    p->scratch_bytes = (fp16 ? 4 : sizeof(float2)) * (size_t)n * (size_t)batch;
     return ESTL_OK;
 }
 
-static int ensure_scratch(Plan *p, int which) {
+int ensure_scratch(Plan *p, int which) {
     if (p->scratch[which]) return ESTL_OK;
     HIP_TRY(hipMalloc(&p->scratch[which], p->scratch_bytes));
     return ESTL_OK;
@@ -252,6 +260,8 @@ int execute_plan(Plan *p, const float2 *in, float2 *out, hipStream_t st) {
 // allocated on first use).  ws1 is needed only for in-place odd-length chains.
 int execute_plan_ws(Plan *p, const float2 *in, float2 *out, float2 *ws0, float2 *ws1, hipStream_t st) {
     if (!in || !out) return set_error(ESTL_ERR_INVALID, "null buffer");
+    if (p->precision != ESTL_PRECISION_FP32)
+        return set_error(ESTL_ERR_INVALID, "an FP16-storage plan runs through estl_execute_fp16");
     switch (p->kind) {
         case Plan::COPY: {
             if (p->scale == 1.0f && in != out)

@@ -53,6 +53,19 @@ __device__ __forceinline__ void st_stream(float2 *p, float2 x) {
 __device__ __forceinline__ float ld_stream_f(const float *p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st_stream_f(float *p, float x) { __builtin_nontemporal_store(x, p); }
 
+// FP16 storage of a complex value (re, im as IEEE binary16), computed on in FP32.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_elem(const float2 *p) { return ld_stream(p); }
+__device__ __forceinline__ void st_elem(float2 *p, float2 x) { st_stream(p, x); }
+__device__ __forceinline__ float2 ld_elem(const f16x2 *p) {
+    const f16x2 h = __builtin_nontemporal_load(p);
+    return make_float2((float)h.x, (float)h.y);
+}
+__device__ __forceinline__ void st_elem(f16x2 *p, float2 x) {
+    const f16x2 h = {(_Float16)x.x, (_Float16)x.y};  // round to nearest even
+    __builtin_nontemporal_store(h, p);
+}
+
 // Returns p unchanged, but hides from the optimiser that it is loop-invariant.  Used on
 // twiddle-table pointers inside loops over rows/items: otherwise every twiddle load of a
 // pass is hoisted out of the loop and ~30 float2 stay live for the whole kernel, which

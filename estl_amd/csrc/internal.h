@@ -34,6 +34,15 @@ hipError_t launch_rows(int log2n, int dir, const float2 *in, float2 *out, long b
                        hipStream_t st);
 hipError_t launch_pass_fwd(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st);
 hipError_t launch_pass_bwd(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st);
+// FP16 storage (fp16.hip): buffers of (re, im) binary16 pairs, passed untyped
+hipError_t launch_pack_fp16(const float2 *x, void *y16, float *scale, int log2n, long batch, hipStream_t st);
+hipError_t launch_unpack_fp16(const void *y16, const float *scale, float2 *x, int log2n, long batch, hipStream_t st);
+hipError_t launch_scale_mul(const float *in, float *out, float f, long batch, hipStream_t st);
+hipError_t launch_rows_fp16(int log2n, int dir, const void *in16, void *out16, long batch, const float2 *tw, float scale,
+                            hipStream_t st);
+hipError_t launch_pass_fp16(int log2b, int log2t, int dir, const void *in16, void *out16, const PassArgs &a,
+                            hipStream_t st);
+int max_pass_log2_fp16();
 // forward last pass + spectrum multiply + inverse first pass of a convolution
 hipError_t launch_pass_mid(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st);
 inline hipError_t launch_pass(int log2b, int log2t, int dir, const float2 *in, float2 *out, const PassArgs &a,

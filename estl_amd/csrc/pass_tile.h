@@ -42,9 +42,13 @@ struct PassTile {
 // pre-twiddle) loads exactly x_s = in[j + s n/B] for its column j.  So the tile
 // multiplies its B bins by the spectrum and runs the inverse radix-B DFT in registers,
 // saving one HBM round trip and one launch.
-template <int LOG2B, int DIR, int LOG2T, bool MID = false>
-__device__ __forceinline__ void pass_tile(const float2 *__restrict__ in, float2 *__restrict__ out, const PassArgs &a,
+//
+// ST is the storage type of `in` and `out`: float2, or f16x2 (FP16 storage, FP32
+// arithmetic; the real-pair I/O modes exist for float2 only).
+template <int LOG2B, int DIR, int LOG2T, bool MID = false, typename ST = float2>
+__device__ __forceinline__ void pass_tile(const ST *__restrict__ in, ST *__restrict__ out, const PassArgs &a,
                                           long tile, float2 *lds, bool lds_busy) {
+    constexpr bool F32 = __is_same(ST, float2);
     using G = BlockGeom<LOG2B>;
     constexpr int E = G::E, TPT = G::TPT, T = 1 << LOG2T, NT = T * TPT;
     constexpr int LDSP = PassTile<LOG2B, LOG2T>::ldsp;
@@ -55,10 +59,10 @@ __device__ __forceinline__ void pass_tile(const float2 *__restrict__ in, float2 
     const int j0 = int(tile - (row << (lcpr - LOG2T))) << LOG2T;
     const int j = j0 + c;
     const int g = j >> ls, z = j & ((1 << ls) - 1);
-    const float2 *src = in + (row << a.log2n);
+    const ST *src = in + (row << a.log2n);
     float2 v[E];
     bool busy = lds_busy;
-    if (a.io_in == 1) {
+    if (F32 && a.io_in == 1) {
         // real pair: rows 2 row and 2 row + 1 of rin (the second absent past the end);
         // only the first pass reads pairs and its plans keep S >= T
         const float *x1 = a.rin + ((2 * row) << a.log2n);
@@ -70,17 +74,17 @@ __device__ __forceinline__ void pass_tile(const float2 *__restrict__ in, float2 
             v[i] = make_float2(ld_stream_f(x1 + o), two ? ld_stream_f(x1 + (1L << a.log2n) + o) : 0.0f);
         }
     } else if (ls >= LOG2T) {
-        const float2 *p = src + ((g << (ls + LOG2B)) + z);
+        const ST *p = src + ((g << (ls + LOG2B)) + z);
 #pragma unroll
-        for (int i = 0; i < E; ++i) v[i] = ld_stream(p + ((tl + TPT * i) << ls));
+        for (int i = 0; i < E; ++i) v[i] = ld_elem(p + ((tl + TPT * i) << ls));
     } else {
         // contiguous block of T*B values starting at B*j0: value o belongs to column
         // (o / (S B)) S + o mod S, element (o mod S B) / S
-        const float2 *p = src + (j0 << LOG2B);
+        const ST *p = src + (j0 << LOG2B);
         const int sbmask = (1 << (ls + LOG2B)) - 1, smask = (1 << ls) - 1;
         float2 r[E];
 #pragma unroll
-        for (int k = 0; k < E; ++k) r[k] = ld_stream(p + t + NT * k);
+        for (int k = 0; k < E; ++k) r[k] = ld_elem(p + t + NT * k);
         if (busy) __syncthreads();
 #pragma unroll
         for (int k = 0; k < E; ++k) {
@@ -106,7 +110,7 @@ __device__ __forceinline__ void pass_tile(const float2 *__restrict__ in, float2 
         }
     }
     block_fft<LOG2B, DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, busy);
-    if (a.io_out == 2) {
+    if (F32 && a.io_out == 2) {
         float *y1 = a.rout + ((2 * row) << a.log2n) + j;
         const bool two = 2 * row + 1 < a.real_rows;
 #pragma unroll
@@ -126,9 +130,9 @@ __device__ __forceinline__ void pass_tile(const float2 *__restrict__ in, float2 
         }
         block_fft<LOG2B, -DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, true);
     }
-    float2 *dst = out + (row << a.log2n) + j;
+    ST *dst = out + (row << a.log2n) + j;
 #pragma unroll
-    for (int i = 0; i < E; ++i) st_stream(dst + ((tl + TPT * i) << lcpr), cscale(v[i], a.scale));
+    for (int i = 0; i < E; ++i) st_elem(dst + ((tl + TPT * i) << lcpr), cscale(v[i], a.scale));
 }
 
 }  // namespace estl

@@ -24,6 +24,7 @@ struct Plan {
     int dir = -1, precision = 0, log2n = 0;
     unsigned flags = 0;
     float scale = 1.0f;
+    double out_factor = 1.0;  // FP16 storage: output scale = input scale * out_factor
     Kind kind = COPY;
     const float2 *tw_rows = nullptr;
     std::vector<PassStep> steps;
@@ -43,6 +44,7 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
                const std::vector<int> *tiles = nullptr);
 int execute_plan(Plan *p, const float2 *in, float2 *out, hipStream_t st);
 int execute_plan_ws(Plan *p, const float2 *in, float2 *out, float2 *ws0, float2 *ws1, hipStream_t st);
+int ensure_scratch(Plan *p, int which);
 void destroy_plan(Plan *p);
 std::string describe_plan(const Plan *p);
 int plan_launches(const Plan *p);

@@ -97,11 +97,30 @@ int estl_tune(long n, long batch, int direction, char *report, int len);
 int estl_plan_destroy(estl_plan_t plan);
 
 /* Asynchronous execute on device buffers: d_in and d_out hold batch*n interleaved
- * complex64 values (batch*n half2 values plus batch floats of scale in FP16 mode, see
- * estl_plan_fp16_scales).  stream is a hipStream_t (NULL = the default stream).
- * d_in == d_out is allowed.  Replaces the pass-launch loop of
- * FFT_OpenCL_Contiguous::execute (FFT_OpenCL_Contiguous.h:108-124). */
+ * complex64 values (FP32 plans; FP16-storage plans run through estl_execute_fp16).
+ * stream is a hipStream_t (NULL = the default stream).  d_in == d_out is allowed.
+ * Replaces the pass-launch loop of FFT_OpenCL_Contiguous::execute
+ * (FFT_OpenCL_Contiguous.h:108-124). */
 int estl_execute_device(estl_plan_t plan, const void *d_in, void *d_out, void *stream);
+
+/* ---------------------------------------------------------------- FP16 storage
+ * No counterpart in the reference, which stores cl_float2 only
+ * (FFT_OpenCL_Contiguous.h:47).  A buffer is batch*n binary16 (re, im) pairs plus one
+ * FP32 scale per transform: true value = stored value * scale.  Arithmetic is FP32;
+ * each pass of radix B stores 1/sqrt(B) of its result, so no stored value can leave
+ * FP16's range for n <= 2^30. */
+
+/* complex64 -> FP16 pairs with per-transform max-abs scaling: scale[t] = max over the
+ * transform of max(|re|, |im|) (1 if all zero), stored = x / scale[t]. */
+int estl_pack_fp16(const void *d_x, void *d_x16, float *d_scale, long n, long batch, void *stream);
+
+/* FP16 pairs -> complex64: x = stored * scale[t]. */
+int estl_unpack_fp16(const void *d_x16, const float *d_scale, void *d_x, long n, long batch, void *stream);
+
+/* Execute a plan created with ESTL_PRECISION_FP16 on FP16 buffers; writes the output
+ * scales (input scales * sqrt(n), times 1/n with ESTL_FLAG_NORMALIZE). */
+int estl_execute_fp16(estl_plan_t plan, const void *d_in, const float *d_in_scale, void *d_out, float *d_out_scale,
+                      void *stream);
 
 /* Synchronous execute on host buffers with the reference's semantics: copy in, run,
  * copy out, wait (FFT_OpenCL_Contiguous::execute, FFT_OpenCL_Contiguous.h:97-142, and
