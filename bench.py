@@ -130,6 +130,58 @@ def bench_uneven(torch, estl_amd, stream, reps: int = 10):
             "tone_rel_l2_max": worst, "work_items": ub.info["items"], "workgroups": ub.info["workgroups"]}
 
 
+def bench_large(torch, estl_amd, stream, reps: int = 10, check: bool = True):
+    """One N = 2^26 transform (seed 2): FP32 multi-pass plan (default split and the
+    autotuned one) and the FP16-storage plan, each timed as the median of `reps`
+    launches; relative L2 error of each result against the FP64 oracle."""
+    import numpy as np
+    import oracle
+    n = 1 << 26
+    x = torch.empty(n, dtype=torch.complex64, device="cuda:0")
+    y = torch.empty_like(x)
+    estl_amd.fill_uniform(x, 2, stream=stream)
+    stream.synchronize()
+
+    def timed(fn):
+        fn()
+        stream.synchronize()
+        ev = []
+        for _ in range(reps):
+            s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.record(stream)
+            fn()
+            e.record(stream)
+            ev.append((s, e))
+        stream.synchronize()
+        return sorted(s.elapsed_time(e) for s, e in ev)[reps // 2]
+
+    out = {"workload": "single C2C forward, N=2^26, uniform[-1,1) seed 2, HBM-resident"}
+    ref = None
+    if check:
+        ref = oracle.fft64(x.cpu().numpy().astype(np.complex128), -1)
+    for name, flags in (("fp32_default", 0), ("fp32_tuned", estl_amd.FLAG_AUTOTUNE)):
+        p = estl_amd.Plan(n, 1, estl_amd.FORWARD, estl_amd.FP32, flags)
+        ms = timed(lambda: p.execute(x, y, stream))
+        out[name] = {"ms": round(ms, 4), "gbs": round(16.0 * n / (ms * 1e-3) / 1e9, 1),
+                     "passes": [[s["radix"], s["tile"]] for s in p.describe().get("passes", [])],
+                     "rel_l2_vs_fp64": oracle.rel_l2(y.cpu().numpy(), ref) if check else None}
+    x16 = torch.empty(2 * n, dtype=torch.float16, device="cuda:0")
+    y16 = torch.empty_like(x16)
+    s_in = torch.empty(1, dtype=torch.float32, device="cuda:0")
+    s_out = torch.empty_like(s_in)
+    estl_amd.pack_fp16(x, x16, s_in, n, 1, stream)
+    p16 = estl_amd.Plan(n, 1, estl_amd.FORWARD, estl_amd.FP16)
+    ms = timed(lambda: p16.execute_fp16(x16, s_in, y16, s_out, stream))
+    estl_amd.unpack_fp16(y16, s_out, y, n, 1, stream)
+    stream.synchronize()
+    out["fp16_storage"] = {"ms": round(ms, 4), "gbs": round(8.0 * n / (ms * 1e-3) / 1e9, 1),
+                           "fp32_equivalent_gbs": round(16.0 * n / (ms * 1e-3) / 1e9, 1),
+                           "scaling": "per-transform max-abs input scale, 1/sqrt(radix) per pass",
+                           "rel_l2_vs_fp64": oracle.rel_l2(y.cpu().numpy(), ref) if check else None}
+    out["bound_fp32"] = 1e-6 * 26
+    return out
+
+
 def _drive(srv, reqs, xp, hp, arrivals, filters=None, keep=()):
     """Submit request i at time arrivals[i] (s from start) -- with its registered filter
     when `filters` maps (n, k) to one, else with the filter inline -- and release each
@@ -290,6 +342,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-size batch leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served convolution stream leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the N = 2^26 (FP32 and FP16 storage) leg")
     ap.add_argument("--percall-rows", type=int, default=BATCH)
     ap.add_argument("--graph", action="store_true",
                     help="replay the step's two launches as one HIP graph (measured: no gain over eager launches)")
@@ -429,6 +482,7 @@ def main() -> int:
 
     uneven = None if args.no_uneven else bench_uneven(torch, estl_amd, stream)
     serve = None if args.no_serve else bench_serve(torch, estl_amd, stream)
+    large = None if args.no_large else bench_large(torch, estl_amd, stream)
 
     cpu = None if args.no_cpu else cpu_baseline(BATCH, 3)
 
@@ -464,6 +518,7 @@ def main() -> int:
                            "bound": 1e-6 * math.log2(N)},
         "uneven_batch": uneven,
         "served_convolution": serve,
+        "large_n": large,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -315,14 +315,17 @@ void destroy_plan(Plan *p) {
 std::string describe_plan(const Plan *p) {
     char buf[256];
     std::string s = "{";
-    snprintf(buf, sizeof buf, "\"n\": %ld, \"batch\": %ld, \"direction\": %d, ", p->n, p->batch, p->dir);
+    snprintf(buf, sizeof buf, "\"n\": %ld, \"batch\": %ld, \"direction\": %d, \"precision\": \"%s\", ", p->n,
+             p->batch, p->dir, p->precision == ESTL_PRECISION_FP16 ? "fp16" : "fp32");
     s += buf;
+    const bool h = p->precision == ESTL_PRECISION_FP16;
     if (p->kind == Plan::COPY) {
         s += "\"kind\": \"copy\", \"passes\": []}";
         return s;
     }
     if (p->kind == Plan::ROWS) {
-        s += "\"kind\": \"rows\", \"kernel\": \"k_fft_rows\", \"radices\": [";
+        s += h ? "\"kind\": \"rows\", \"kernel\": \"k_fft_rows_h\", \"radices\": ["
+               : "\"kind\": \"rows\", \"kernel\": \"k_fft_rows\", \"radices\": [";
         for (int q = 0; q < rows_pass_count(p->log2n); ++q) {
             long A;
             int B;
@@ -333,7 +336,8 @@ std::string describe_plan(const Plan *p) {
         s += "], \"launches\": 1}";
         return s;
     }
-    s += "\"kind\": \"passes\", \"kernel\": \"k_fft_pass\", \"passes\": [";
+    s += h ? "\"kind\": \"passes\", \"kernel\": \"k_fft_pass_h\", \"passes\": ["
+           : "\"kind\": \"passes\", \"kernel\": \"k_fft_pass\", \"passes\": [";
     for (size_t q = 0; q < p->steps.size(); ++q) {
         const auto &st = p->steps[q];
         snprintf(buf, sizeof buf, "%s{\"radix\": %ld, \"A\": %ld, \"S\": %ld, \"tile\": %d}", q ? ", " : "",
