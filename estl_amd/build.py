@@ -71,6 +71,23 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+def build_cpp_check(verbose: bool = True) -> str:
+    """tests/cpp/dropin_check.cpp against this repository's C++ headers (include/estl),
+    linked to the library: the C++ drop-in layer exercised by the GPU test suite."""
+    repo = os.path.dirname(HERE)
+    out = os.path.join(repo, "tests", "cpp", "dropin_check_own")
+    cmd = ["g++", "-O1", "-std=c++17", "-Wall", "-I", os.path.join(repo, "include"),
+           "-I", os.path.join(repo, "include", "estl"), "-I", "/opt/rocm/include",
+           os.path.join(repo, "tests", "cpp", "dropin_check.cpp"), "-o", out,
+           "-L", HERE, "-lestl_hip", "-Wl,-rpath,$ORIGIN/../../estl_amd", "-L/opt/rocm/lib",
+           "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return out
+
+
 if __name__ == "__main__":
     build()
+    build_cpp_check()
     sys.exit(0)

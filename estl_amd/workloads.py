@@ -81,8 +81,10 @@ def fill_uneven(batch: UnevenBatch, buf, torch) -> None:
 class ConvRequests:
     n: np.ndarray          # int64, row length 2^U{log2n_min..log2n_max}
     rows: np.ndarray       # int64, rows per request, U{1..rows_max} capped at max_values / n
-    k: np.ndarray          # int64, filter length 2^U{log2k_min..log2k_max}
+    k: np.ndarray          # int64, filter length U{k_min..k_max}
     flags: np.ndarray      # int64, 0 or correlation (correlate_fraction of requests)
+    bank: np.ndarray       # int64, index into a bank of `bank_size` filters (bank mode)
+    bank_k: np.ndarray     # int64, lengths of the bank's filters, U{k_min..k_max}
     gaps: np.ndarray       # float64, unit-rate exponential inter-arrival gaps
     seed: int
 
@@ -90,18 +92,22 @@ class ConvRequests:
         return self.n * self.rows
 
 
-def conv_requests(count: int = 1000, seed: int = 5, log2n=(10, 22), log2k=(4, 13), rows_max: int = 256,
-                  max_values: int = 1 << 22, correlate_fraction: float = 0.25) -> ConvRequests:
+def conv_requests(count: int = 1000, seed: int = 4, log2n=(10, 22), k_range=(16, 8192), rows_max: int = 256,
+                  max_values: int = 1 << 22, correlate_fraction: float = 0.25, bank_size: int = 32) -> ConvRequests:
     """The served convolution stream: request i convolves rows[i] real rows of n[i]
-    values with a k[i]-value filter; arrivals are a Poisson process (exponential gaps of
-    unit mean, scaled by 1 / rate by the driver)."""
+    values with its own k[i]-value filter (or, in bank mode, with filter bank[i] of a
+    fixed bank); arrivals are a Poisson process (exponential gaps of unit mean, scaled
+    by 1 / rate by the driver)."""
     import estl_amd
     ln = log2n[0] + (stream(seed, count) % np.uint64(log2n[1] - log2n[0] + 1)).astype(np.int64)
-    lk = log2k[0] + (stream(seed ^ 0x4B, count) % np.uint64(log2k[1] - log2k[0] + 1)).astype(np.int64)
+    span = np.uint64(k_range[1] - k_range[0] + 1)
+    k = k_range[0] + (stream(seed ^ 0x4B, count) % span).astype(np.int64)
     n = np.left_shift(np.int64(1), ln)
     rows = 1 + (stream(seed ^ 0x52, count) % np.uint64(rows_max)).astype(np.int64)
     rows = np.minimum(rows, np.maximum(1, max_values // n))
     corr = uniform01(seed ^ 0xC0, count) < correlate_fraction
     flags = np.where(corr, estl_amd.CONV_CORRELATE, 0).astype(np.int64)
+    bank = (stream(seed ^ 0xBA, count) % np.uint64(bank_size)).astype(np.int64)
+    bank_k = k_range[0] + (stream(seed ^ 0xBB, bank_size) % span).astype(np.int64)
     gaps = -np.log1p(-uniform01(seed ^ 0x9A, count))
-    return ConvRequests(n=n, rows=rows, k=np.left_shift(np.int64(1), lk), flags=flags, gaps=gaps, seed=seed)
+    return ConvRequests(n=n, rows=rows, k=k, flags=flags, bank=bank, bank_k=bank_k, gaps=gaps, seed=seed)

@@ -21,8 +21,11 @@
 #include <stdexcept>
 #include <string>
 
-#include "../estl_hip.h"
-#include "FFT.h"
+// Angle brackets on purpose: with the ESTL tree ahead of include/estl on the include
+// path this compiles against ESTL's own FFT.h / DataInterface.h / Complex.h; without
+// it, against the API-identical copies in include/estl.  Needs include/ on the path.
+#include <FFT.h>
+#include <estl_hip.h>
 
 template <class D>
 class FFT_HIP;
