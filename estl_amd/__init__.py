@@ -75,6 +75,10 @@ def lib():
     _sig(L, "estl_plan_describe", ci, [vp, ctypes.c_char_p, ci])
     _sig(L, "estl_plan_launches", ci, [vp])
     _sig(L, "estl_fill_uniform", ci, [vp, cl, u64, u64, vp])
+    _sig(L, "estl_plan_profile", ci, [vp, vp, vp, ci, ctypes.c_char_p, ci])
+    _sig(L, "estl_plan_kernel_flops", ctypes.c_double, [vp, ci])
+    _sig(L, "estl_plan_kernel_ms", ctypes.c_double, [vp, ci])
+    _sig(L, "estl_plan_kernel_info", ci, [vp, ci, ctypes.c_char_p, ci])
     _sig(L, "estl_tune", ci, [cl, cl, ci, ctypes.c_char_p, ci])
     lp = ctypes.POINTER(ctypes.c_long)
     _sig(L, "estl_ubatch_create", ci, [ctypes.POINTER(vp), ci, lp, lp, ci, cu])
@@ -82,6 +86,11 @@ def lib():
     _sig(L, "estl_ubatch_execute_graph", ci, [vp, vp, vp, vp])
     _sig(L, "estl_ubatch_describe", ci, [vp, ctypes.c_char_p, ci])
     _sig(L, "estl_ubatch_destroy", ci, [vp])
+    _sig(L, "estl_plan2d_create", ci, [ctypes.POINTER(vp), cl, cl, cl, ci, cu])
+    _sig(L, "estl_plan2d_execute", ci, [vp, vp, vp, vp])
+    _sig(L, "estl_plan2d_execute_host", ci, [vp, vp, vp])
+    _sig(L, "estl_plan2d_launches", ci, [vp])
+    _sig(L, "estl_plan2d_destroy", ci, [vp])
     _sig(L, "estl_pack_fp16", ci, [vp, vp, vp, cl, cl, vp])
     _sig(L, "estl_unpack_fp16", ci, [vp, vp, vp, cl, cl, vp])
     _sig(L, "estl_execute_fp16", ci, [vp, vp, vp, vp, vp, vp])
@@ -170,6 +179,15 @@ class Plan:
         check(lib().estl_plan_describe(self._h, buf, 4096))
         return json.loads(buf.value.decode())
 
+    def profile(self, reps: int = 10, d_in=None, d_out=None) -> dict:
+        """Time each launch of the plan (on its own buffers unless given): per-launch
+        ms, flops (5 n log2 radix per transform), GFLOP/s and share, as JSON."""
+        pin = None if d_in is None else (d_in if isinstance(d_in, int) else d_in.data_ptr())
+        pout = None if d_out is None else (d_out if isinstance(d_out, int) else d_out.data_ptr())
+        buf = ctypes.create_string_buffer(1 << 14)
+        check(lib().estl_plan_profile(self._h, pin, pout, reps, buf, 1 << 14))
+        return json.loads(buf.value.decode())
+
     def launches(self) -> int:
         return check(lib().estl_plan_launches(self._h))
 
@@ -179,6 +197,42 @@ class Plan:
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
             lib().estl_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan2D:
+    """2-D transforms of `batch` ny x nx arrays (x fastest; C ABI estl_plan2d_*)."""
+
+    def __init__(self, nx: int, ny: int, batch: int = 1, direction: int = FORWARD, flags: int = 0):
+        self.nx, self.ny, self.batch, self.direction, self.flags = nx, ny, batch, direction, flags
+        self._h = ctypes.c_void_p()
+        h = ctypes.c_void_p()
+        check(lib().estl_plan2d_create(ctypes.byref(h), nx, ny, batch, direction, flags))
+        self._h = h
+
+    def execute(self, d_in, d_out, stream=None) -> None:
+        for t in (d_in, d_out):
+            if not isinstance(t, int) and t.numel() * t.element_size() < 8 * self.nx * self.ny * self.batch:
+                raise ValueError("buffer smaller than batch * ny * nx complex64")
+        pin = d_in if isinstance(d_in, int) else d_in.data_ptr()
+        pout = d_out if isinstance(d_out, int) else d_out.data_ptr()
+        check(lib().estl_plan2d_execute(self._h, pin, pout, _stream_ptr(stream)))
+
+    def execute_host(self, h_in, h_out) -> None:
+        check(lib().estl_plan2d_execute_host(self._h, h_in.ctypes.data, h_out.ctypes.data))
+
+    def launches(self) -> int:
+        return check(lib().estl_plan2d_launches(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().estl_plan2d_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

@@ -36,7 +36,9 @@ SOURCES = [("kernels.hip", [], "kernels"),
            ("fp16_c.cpp", [], "fp16_c"),
            ("ubatch_c.cpp", [], "ubatch_c"),
            ("conv_c.cpp", [], "conv_c"),
-           ("server.cpp", [], "server")]
+           ("server.cpp", [], "server"),
+           ("profile.cpp", [], "profile"),
+           ("plan2d.cpp", [], "plan2d")]
 
 
 def _compile(item, verbose: bool) -> str:

@@ -41,13 +41,7 @@ int set_error(int code, const char *fmt, ...) {
 }
 const char *last_error() { return g_last_error.c_str(); }
 
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess)                                                                      \
-            return set_error(ESTL_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
-                             __FILE__, __LINE__);                                                  \
-    } while (0)
+
 
 // ------------------------------------------------------------------ device state
 static thread_local int g_device = -1;
@@ -203,7 +197,9 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
     int total = 0;
     const int max_b = fp16 ? max_pass_log2_fp16() : kMaxPassLog2;
     for (int b : bits) {
-        if (b < 4 || b > max_b) return set_error(ESTL_ERR_UNSUPPORTED, "pass radix 2^%d outside [2^4, 2^%d]", b, max_b);
+        const int min_b = fp16 ? 4 : 1;
+        if (b < min_b || b > max_b)
+            return set_error(ESTL_ERR_UNSUPPORTED, "pass radix 2^%d outside [2^%d, 2^%d]", b, min_b, max_b);
         total += b;
     }
     if (total != p->log2n) return set_error(ESTL_ERR_INVALID, "pass split does not sum to log2n=%d", p->log2n);
@@ -280,10 +276,12 @@ int execute_plan_ws(Plan *p, const float2 *in, float2 *out, float2 *ws0, float2 
                 ws0 = p->scratch[0];
             }
             // destinations backwards from out, alternating with scratch 0; the first pass
-            // must not write its own input, so an in-place odd-length chain uses scratch 1
+            // of a longer chain must not write its own input, so an in-place odd-length
+            // chain starts in scratch 1 (a single pass may run in place: every tile
+            // writes exactly the positions it read)
             std::vector<float2 *> dst(P);
             for (int q = P - 1, k = 0; q >= 0; --q, ++k) dst[q] = (k & 1) ? ws0 : out;
-            if (dst[0] == in) {
+            if (P > 1 && dst[0] == in) {
                 if (!ws1) {
                     rc = ensure_scratch(p, 1);
                     if (rc) return rc;

@@ -37,7 +37,7 @@ int execute_plan_fp16(Plan *p, const void *in, const float *in_scale, void *out,
             if (rc) return rc;
             std::vector<void *> dst(P);
             for (int q = P - 1, k = 0; q >= 0; --q, ++k) dst[q] = (k & 1) ? (void *)p->scratch[0] : out;
-            if (dst[0] == in) {
+            if (P > 1 && dst[0] == in) {  // a single pass may run in place
                 rc = ensure_scratch(p, 1);
                 if (rc) return rc;
                 dst[0] = p->scratch[1];

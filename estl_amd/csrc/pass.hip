@@ -65,6 +65,9 @@ static hipError_t launch_b(int log2t, const float2 *in, float2 *out, const PassA
 hipError_t ESTL_PASS_LAUNCHER(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a,
                               hipStream_t st) {
     switch (log2b) {
+        case 1: return launch_b<1>(log2t, in, out, a, st);
+        case 2: return launch_b<2>(log2t, in, out, a, st);
+        case 3: return launch_b<3>(log2t, in, out, a, st);
         case 4: return launch_b<4>(log2t, in, out, a, st);
         case 5: return launch_b<5>(log2t, in, out, a, st);
         case 6: return launch_b<6>(log2t, in, out, a, st);

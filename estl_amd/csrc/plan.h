@@ -33,9 +33,21 @@ struct Plan {
     // resident buffers and stream of the synchronous host entry point
     float2 *h_in_dev = nullptr, *h_out_dev = nullptr;
     hipStream_t stream = nullptr;
+    // mean time of each launch from the last estl_plan_profile
+    std::vector<double> kernel_ms;
 };
 
 int set_error(int code, const char *fmt, ...);
+
+// return ESTL_ERR_HIP with the failing call's text from the enclosing function
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return set_error(ESTL_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                             __FILE__, __LINE__);                                                  \
+    } while (0)
+
 int current_device();
 int block_table(int device, int log2n, const float2 **out);
 int linear_table(int device, long count, long mult, long nq, const float2 **out);

@@ -60,14 +60,23 @@ class FFT_HIP<float> : public FFT<float> {
         if (rc != ESTL_OK) printf("FFT_HIP::execute failed: %s\n", estl_last_error());
     }
 
-    // profiling surface in the spirit of the reference's per-kernel report
-    // (FFT_OpenCL_Contiguous.h:94-95, tests/Test.cpp:86-100)
+    // profiling surface of the reference's OpenCL backends (kernelTimers,
+    // getTotalComputationFlops(q), getKernelInfo(q): FFT_OpenCL_Contiguous.h:73-78,94-95)
+    // as tests/Test.cpp:86-100 prints it: profile() times every launch on the plan's
+    // resident buffers, then getKernelTime(q) holds its mean in milliseconds
     int getKernelCount() const { return estl_plan_launches(plan); }
     double getTotalComputationFlops() const {
         int l = 0;
         while ((1 << l) < size) ++l;
         return 5.0 * size * l * batchCount;
     }
+    double getTotalComputationFlops(int q) const { return estl_plan_kernel_flops(plan, q); }
+    std::string getKernelInfo(int q) const {
+        char buf[256];
+        return estl_plan_kernel_info(plan, q, buf, sizeof buf) < 0 ? std::string() : std::string(buf);
+    }
+    double getKernelTime(int q) const { return estl_plan_kernel_ms(plan, q); }
+    bool profile(int reps = 10) { return estl_plan_profile(plan, nullptr, nullptr, reps, nullptr, 0) >= 0; }
     std::string describe() const {
         char buf[2048];
         estl_plan_describe(plan, buf, sizeof buf);

@@ -56,6 +56,7 @@ typedef struct estl_plan_s *estl_plan_t;
 typedef struct estl_ubatch_s *estl_ubatch_t;
 typedef struct estl_conv_s *estl_conv_t;
 typedef struct estl_server_s *estl_server_t;
+typedef struct estl_plan2d_s *estl_plan2d_t;
 
 /* Message of the calling thread's last failure ("" if none). */
 const char *estl_last_error(void);
@@ -103,6 +104,19 @@ int estl_plan_destroy(estl_plan_t plan);
  * (FFT_OpenCL_Contiguous.h:108-124). */
 int estl_execute_device(estl_plan_t plan, const void *d_in, void *d_out, void *stream);
 
+/* ---------------------------------------------------------------- 2-D transforms
+ * `batch` arrays of ny rows of nx complex64, x fastest (FFT2D's data[y * xDim + x],
+ * FFT.h:54-84, which runs one host round trip per row and per column).  Rows are one
+ * batched 1-D launch; columns are Stockham passes at stride nx (no transpose; one pass
+ * for ny <= 2^13), so the transform makes two HBM round trips.  ESTL_FLAG_NORMALIZE
+ * scales by 1/(nx ny).  d_in == d_out is allowed. */
+int estl_plan2d_create(estl_plan2d_t *plan, long nx, long ny, long batch, int direction, unsigned flags);
+int estl_plan2d_execute(estl_plan2d_t plan, const void *d_in, void *d_out, void *stream);
+/* Synchronous host-buffer form (copy in, transform, copy out), like FFT2D::execute. */
+int estl_plan2d_execute_host(estl_plan2d_t plan, const void *h_in, void *h_out);
+int estl_plan2d_launches(estl_plan2d_t plan);
+int estl_plan2d_destroy(estl_plan2d_t plan);
+
 /* ---------------------------------------------------------------- FP16 storage
  * No counterpart in the reference, which stores cl_float2 only
  * (FFT_OpenCL_Contiguous.h:47).  A buffer is batch*n binary16 (re, im) pairs plus one
@@ -139,6 +153,23 @@ long long estl_plan_workspace_bytes(estl_plan_t plan);
  * Returns the full length; the reference exposes the per-pass radix as getKernelInfo
  * (FFT_OpenCL_Contiguous.h:95). */
 int estl_plan_describe(estl_plan_t plan, char *buf, int len);
+
+/* Profile: run each launch of the plan on its own between two HIP events, `reps` times
+ * after a warm-up (d_in/d_out NULL: the plan's own resident buffers), keep the mean per
+ * launch and write a JSON report (per launch: info, ms, flops, GFLOP/s, share).  The
+ * reference's kernelTimers / getTotalComputationFlops(q) / getKernelInfo(q) surface
+ * (FFT_OpenCL_Contiguous.h:73-78,94-95; uOpenCL.h:281-282,339-340) as printed by
+ * tests/Test.cpp:86-100.  Returns the report's length. */
+int estl_plan_profile(estl_plan_t plan, const void *d_in, void *d_out, int reps, char *json, int len);
+
+/* Flops of launch q: 5 n log2(radix_q) per transform (the rows kernel: 5 n log2 n). */
+double estl_plan_kernel_flops(estl_plan_t plan, int q);
+
+/* Mean milliseconds of launch q from the last estl_plan_profile (-1 if not profiled). */
+double estl_plan_kernel_ms(estl_plan_t plan, int q);
+
+/* Description of launch q (kernel, radix, tile). */
+int estl_plan_kernel_info(estl_plan_t plan, int q, char *buf, int len);
 
 /* Number of kernel launches one execute makes (the reference: one per pass,
  * FFT_OpenCL_Contiguous.h:108). */

@@ -18,8 +18,12 @@
 //      reduced angles) -- the role the FFTW3 factory plays there -- |diff|^2 <= 1e-3 Mx My,
 //      on seeded random input (the reference's input degenerates to a ramp);
 //   3. a backward transform of the forward result gives M times the input on every
-//      batch row (the reference's batched inverse conjugates row 0 only).
+//      batch row (the reference's batched inverse conjugates row 0 only);
+//   4. the per-kernel profiling report in the format of tests/Test.cpp:86-100.
+// The 2-D test also runs FFT2D_HIP (the device-resident 2-D transform) on the same
+// input against the naive factory.
 #include <FFT.h>
+#include <FFT2D_HIP.h>
 #include <FFT_HIP.h>
 
 #include <cmath>
@@ -121,13 +125,21 @@ int main() {
                 a.data[q] = v;
                 r.data[q] = v;
             }
+            FFT2D_HIP<D> d(Mx, My);
+            for (int q = 0; q < Mx * My; ++q) d.data[q] = a.data[q];
             a.execute();
             r.execute();
-            double worst = 0;
-            for (int q = 0; q < Mx * My; ++q) worst = std::max(worst, norm2(a.data[q] - r.data[q]) / (Mx * My));
+            d.execute();
+            double worst = 0, worst_d = 0;
+            for (int q = 0; q < Mx * My; ++q) {
+                worst = std::max(worst, norm2(a.data[q] - r.data[q]) / (Mx * My));
+                worst_d = std::max(worst_d, norm2(d.data[q] - r.data[q]) / (Mx * My));
+            }
             char what[96];
             snprintf(what, sizeof what, "FFT2D differential: %d x %d", Mx, My);
             report(worst <= 1e-3, what, worst, 1e-3);
+            snprintf(what, sizeof what, "FFT2D_HIP (device 2-D) differential: %d x %d", Mx, My);
+            report(worst_d <= 1e-3, what, worst_d, 1e-3);
         }
 
     // 3. batched inverse on every row
@@ -145,6 +157,25 @@ int main() {
         delete f;
         delete g;
         report(worst <= 1e-3, "batched inverse round trip: M=1024 batch=7, every row", worst, 1e-3);
+    }
+
+    // 4. the per-kernel report of tests/Test.cpp:86-100 from the profiling surface
+    for (int m : {12, 16, 20}) {
+        FFT_HIP<D> f(1 << m, true, m == 12 ? 4096 : 16);
+        const bool ok = f.profile(5) && f.getKernelCount() > 0;
+        double totalT = 0, totalOps = 0;
+        for (int q = 0; q < f.getKernelCount(); ++q) totalT += f.getKernelTime(q);
+        printf("  N=2^%d batch=%d:", m, f.batchCount);
+        for (int q = 0; q < f.getKernelCount(); ++q) {
+            const double t = f.getKernelTime(q), ops = f.getTotalComputationFlops(q);
+            totalOps += ops;
+            printf(" [K%d (%s): %.1f Gfs (%d%%)]", q + 1, f.getKernelInfo(q).c_str(), 1e-6 * ops / t,
+                   (int)(100 * (t / totalT) + 0.5));
+        }
+        printf(" (Total: %.1f Gfs)\n", 1e-6 * totalOps / totalT);
+        char what[64];
+        snprintf(what, sizeof what, "profiling surface: N=2^%d", m);
+        report(ok && totalT > 0 && totalOps == f.getTotalComputationFlops(), what, totalT, 1e9);
     }
 
     printf("%s: %d failure(s)\n", failures ? "FAILED" : "OK", failures);
