@@ -20,6 +20,12 @@
 
 namespace estl {
 
+// exact pre-twiddle table reads every kTwRefresh-th element of a thread (see pass_tile)
+#ifndef ESTL_TW_REFRESH
+#define ESTL_TW_REFRESH 4
+#endif
+constexpr int kTwRefresh = ESTL_TW_REFRESH;
+
 template <int LOG2B, int LOG2T>
 struct PassTile {
     static constexpr int B = 1 << LOG2B, T = 1 << LOG2T;
@@ -99,13 +105,27 @@ __device__ __forceinline__ void pass_tile(const ST *__restrict__ in, ST *__restr
         busy = true;
     }
     if (a.A > 1) {
-        // W_{A B}^{g s} = W^{lo} W^{hi << lobits}, m = g s < A B (two exactly rounded
-        // table values, one complex product)
+        // W_{A B}^{g s} for s = tl + TPT i.  A table value is W^{lo} W^{hi << lobits},
+        // m = g s < A B (two exactly rounded entries, one complex product).  Exact
+        // values are taken at i = 0, R, 2R, ...; in between, w_{kR + r} = w_{kR} step^r
+        // with step = W^{g TPT} -- at most two extra roundings per twiddle, and
+        // 2 (E / R + 1) table reads per thread instead of 2 E.
         const unsigned lomask = (1u << a.lobits) - 1;
+        const unsigned gu = unsigned(g);
+        auto table = [&](unsigned m) { return cmul(a.tw_lo[m & lomask], a.tw_hi[m >> a.lobits]); };
+        constexpr int R = (kTwRefresh < E) ? kTwRefresh : E;
+        float2 sp[R];
+        sp[0] = make_float2(1.0f, 0.0f);
+        if constexpr (R > 1) {
+            sp[1] = table(gu * unsigned(TPT));
+#pragma unroll
+            for (int r = 2; r < R; ++r) sp[r] = cmul(sp[r >> 1], sp[r - (r >> 1)]);
+        }
+        float2 base = sp[0];
 #pragma unroll
         for (int i = 0; i < E; ++i) {
-            const unsigned m = unsigned(g) * unsigned(tl + TPT * i);
-            const float2 w = cmul(a.tw_lo[m & lomask], a.tw_hi[m >> a.lobits]);
+            if (i % R == 0) base = table(gu * unsigned(tl + TPT * i));
+            const float2 w = (i % R == 0) ? base : cmul(base, sp[i % R]);
             v[i] = ctw<DIR>(v[i], w);
         }
     }
