@@ -37,8 +37,62 @@ struct PassTile {
     static constexpr int threads = BlockGeom<LOG2B>::TPT << LOG2T;
 };
 
-// Computes tile `tile` of the pass described by `a`.  All threads of the workgroup
-// (PassTile::threads of them) must call it together.  `lds` holds PassTile::lds_size
+// Geometry of tile `tile` for the calling thread: the row, the tile's first column j0,
+// the thread's column j = j0 + c, its g = j / S and z = j mod S, and tl.
+template <int LOG2B, int LOG2T>
+struct TilePos {
+    long row;
+    int j0, j, g, z, c, tl, t;
+    __device__ __forceinline__ TilePos(const PassArgs &a, long tile) {
+        t = opaque_lane(threadIdx.x);
+        c = t & ((1 << LOG2T) - 1);
+        tl = t >> LOG2T;
+        const int lcpr = a.log2n - LOG2B;  // columns per row = 2^lcpr
+        row = tile >> (lcpr - LOG2T);
+        j0 = int(tile - (row << (lcpr - LOG2T))) << LOG2T;
+        j = j0 + c;
+        g = j >> a.log2S;
+        z = j & ((1 << a.log2S) - 1);
+    }
+};
+
+// Phase 1 of a tile: its HBM loads, E values per thread.  Direct path (S >= T) and the
+// real-pair path: the thread's own column, element s = tl + TPT i in v[i].  Staged path
+// (S < T): the thread's share of the tile's input, a contiguous block of T B values
+// starting at B j0 (value t + NT k in v[k]); pass_tile_finish routes it through LDS.
+template <int LOG2B, int LOG2T, typename ST = float2>
+__device__ __forceinline__ void pass_tile_load(const ST *__restrict__ in, const PassArgs &a, long tile,
+                                               float2 (&v)[BlockGeom<LOG2B>::E]) {
+    constexpr bool F32 = __is_same(ST, float2);
+    using G = BlockGeom<LOG2B>;
+    constexpr int E = G::E, TPT = G::TPT, NT = TPT << LOG2T;
+    const TilePos<LOG2B, LOG2T> P(a, tile);
+    const int ls = a.log2S;
+    if (F32 && a.io_in == 1) {
+        // real pair: rows 2 row and 2 row + 1 of rin (the second absent past the end);
+        // only the first pass reads pairs and its plans keep S >= T
+        const float *x1 = a.rin + ((2 * P.row) << a.log2n);
+        const bool two = 2 * P.row + 1 < a.real_rows;
+        const int base = (P.g << (ls + LOG2B)) + P.z;
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int o = base + ((P.tl + TPT * i) << ls);
+            v[i] = make_float2(ld_stream_f(x1 + o), two ? ld_stream_f(x1 + (1L << a.log2n) + o) : 0.0f);
+        }
+    } else if (ls >= LOG2T) {
+        const ST *p = in + (P.row << a.log2n) + ((P.g << (ls + LOG2B)) + P.z);
+#pragma unroll
+        for (int i = 0; i < E; ++i) v[i] = ld_elem(p + ((P.tl + TPT * i) << ls));
+    } else {
+        const ST *p = in + (P.row << a.log2n) + (P.j0 << LOG2B);
+#pragma unroll
+        for (int k = 0; k < E; ++k) v[k] = ld_elem(p + P.t + NT * k);
+    }
+}
+
+// Phase 2 of a tile: (staged path: the LDS exchange that puts each column's values in
+// its threads,) pre-twiddle, radix-B DFT, store.  All threads of the workgroup
+// (PassTile::threads of them) call it together.  `lds` holds PassTile::lds_size
 // float2; if `lds_busy`, a previous user of it may still be reading and a barrier
 // comes first.
 //
@@ -52,52 +106,28 @@ struct PassTile {
 // ST is the storage type of `in` and `out`: float2, or f16x2 (FP16 storage, FP32
 // arithmetic; the real-pair I/O modes exist for float2 only).
 template <int LOG2B, int DIR, int LOG2T, bool MID = false, typename ST = float2>
-__device__ __forceinline__ void pass_tile(const ST *__restrict__ in, ST *__restrict__ out, const PassArgs &a,
-                                          long tile, float2 *lds, bool lds_busy) {
+__device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const PassArgs &a, long tile,
+                                                 float2 (&v)[BlockGeom<LOG2B>::E], float2 *lds, bool lds_busy) {
     constexpr bool F32 = __is_same(ST, float2);
     using G = BlockGeom<LOG2B>;
     constexpr int E = G::E, TPT = G::TPT, T = 1 << LOG2T, NT = T * TPT;
     constexpr int LDSP = PassTile<LOG2B, LOG2T>::ldsp;
-    const int t = opaque_lane(threadIdx.x), c = t & (T - 1), tl = t >> LOG2T;
+    const TilePos<LOG2B, LOG2T> P(a, tile);
     const int ls = a.log2S;
-    const int lcpr = a.log2n - LOG2B;  // columns per row = 2^lcpr
-    const long row = tile >> (lcpr - LOG2T);
-    const int j0 = int(tile - (row << (lcpr - LOG2T))) << LOG2T;
-    const int j = j0 + c;
-    const int g = j >> ls, z = j & ((1 << ls) - 1);
-    const ST *src = in + (row << a.log2n);
-    float2 v[E];
+    const int lcpr = a.log2n - LOG2B;
+    const int c = P.c, tl = P.tl, j = P.j;
     bool busy = lds_busy;
-    if (F32 && a.io_in == 1) {
-        // real pair: rows 2 row and 2 row + 1 of rin (the second absent past the end);
-        // only the first pass reads pairs and its plans keep S >= T
-        const float *x1 = a.rin + ((2 * row) << a.log2n);
-        const bool two = 2 * row + 1 < a.real_rows;
-        const int base = (g << (ls + LOG2B)) + z;
-#pragma unroll
-        for (int i = 0; i < E; ++i) {
-            const int o = base + ((tl + TPT * i) << ls);
-            v[i] = make_float2(ld_stream_f(x1 + o), two ? ld_stream_f(x1 + (1L << a.log2n) + o) : 0.0f);
-        }
-    } else if (ls >= LOG2T) {
-        const ST *p = src + ((g << (ls + LOG2B)) + z);
-#pragma unroll
-        for (int i = 0; i < E; ++i) v[i] = ld_elem(p + ((tl + TPT * i) << ls));
-    } else {
-        // contiguous block of T*B values starting at B*j0: value o belongs to column
-        // (o / (S B)) S + o mod S, element (o mod S B) / S
-        const ST *p = src + (j0 << LOG2B);
+    if (!(F32 && a.io_in == 1) && ls < LOG2T) {
+        // value o of the contiguous block belongs to column (o / (S B)) S + o mod S,
+        // element (o mod S B) / S
         const int sbmask = (1 << (ls + LOG2B)) - 1, smask = (1 << ls) - 1;
-        float2 r[E];
-#pragma unroll
-        for (int k = 0; k < E; ++k) r[k] = ld_elem(p + t + NT * k);
         if (busy) __syncthreads();
 #pragma unroll
         for (int k = 0; k < E; ++k) {
-            const int o = t + NT * k;
+            const int o = P.t + NT * k;
             const int gl = o >> (ls + LOG2B), rem = o & sbmask;
             const int s = rem >> ls, zz = rem & smask;
-            lds[((gl << ls) + zz) * LDSP + Lds<kLdsPad>::idx(s)] = r[k];
+            lds[((gl << ls) + zz) * LDSP + Lds<kLdsPad>::idx(s)] = v[k];
         }
         __syncthreads();
 #pragma unroll
@@ -111,7 +141,7 @@ __device__ __forceinline__ void pass_tile(const ST *__restrict__ in, ST *__restr
         // with step = W^{g TPT} -- at most two extra roundings per twiddle, and
         // 2 (E / R + 1) table reads per thread instead of 2 E.
         const unsigned lomask = (1u << a.lobits) - 1;
-        const unsigned gu = unsigned(g);
+        const unsigned gu = unsigned(P.g);
         auto table = [&](unsigned m) { return cmul(a.tw_lo[m & lomask], a.tw_hi[m >> a.lobits]); };
         constexpr int R = (kTwRefresh < E) ? kTwRefresh : E;
         float2 sp[R];
@@ -131,8 +161,8 @@ __device__ __forceinline__ void pass_tile(const ST *__restrict__ in, ST *__restr
     }
     block_fft<LOG2B, DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, busy);
     if (F32 && a.io_out == 2) {
-        float *y1 = a.rout + ((2 * row) << a.log2n) + j;
-        const bool two = 2 * row + 1 < a.real_rows;
+        float *y1 = a.rout + ((2 * P.row) << a.log2n) + j;
+        const bool two = 2 * P.row + 1 < a.real_rows;
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int o = (tl + TPT * i) << lcpr;
@@ -150,9 +180,18 @@ __device__ __forceinline__ void pass_tile(const ST *__restrict__ in, ST *__restr
         }
         block_fft<LOG2B, -DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, true);
     }
-    ST *dst = out + (row << a.log2n) + j;
+    ST *dst = out + (P.row << a.log2n) + j;
 #pragma unroll
     for (int i = 0; i < E; ++i) st_elem(dst + ((tl + TPT * i) << lcpr), cscale(v[i], a.scale));
+}
+
+// Computes tile `tile` of the pass described by `a` (load, then finish).
+template <int LOG2B, int DIR, int LOG2T, bool MID = false, typename ST = float2>
+__device__ __forceinline__ void pass_tile(const ST *__restrict__ in, ST *__restrict__ out, const PassArgs &a,
+                                          long tile, float2 *lds, bool lds_busy) {
+    float2 v[BlockGeom<LOG2B>::E];
+    pass_tile_load<LOG2B, LOG2T, ST>(in, a, tile, v);
+    pass_tile_finish<LOG2B, DIR, LOG2T, MID, ST>(out, a, tile, v, lds, lds_busy);
 }
 
 }  // namespace estl

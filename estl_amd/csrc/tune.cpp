@@ -240,6 +240,32 @@ int tune_plan(long n, long batch, int dir, bool *rows, std::vector<int> *split, 
         first = false;
         if (us < best.us) best = c;
     }
+    // second stage: tile widths of the best split, one pass at a time (coordinate
+    // descent over 2^-2 .. 2^+1 times the default width; narrow tiles get the XCD remap)
+    if (best.us < 1e30 && !best.rows) {
+        for (size_t q = 0; q < best.split.size(); ++q) {
+            const int b = best.split[q], d = best.tiles[q];
+            for (int lt : {d - 2, d - 1, d + 1}) {
+                if (lt < 0 || lt > kMaxPassCols || lt + b > kMaxPassTileLog2 || lt > log2n - b) continue;
+                Choice c = best;
+                c.tiles[q] = lt;
+                double us = 0;
+                if (time_candidate(c, n, batch, dir, in, out, st, e0, e1, &us)) continue;
+                char buf[256];
+                std::string s = "[";
+                for (size_t k = 0; k < c.split.size(); ++k) s += (k ? ", " : "") + std::to_string(c.split[k]);
+                s += "], \"tiles\": [";
+                for (size_t k = 0; k < c.tiles.size(); ++k) s += (k ? ", " : "") + std::to_string(c.tiles[k]);
+                s += "]";
+                snprintf(buf, sizeof buf, ", {\"plan\": %s, \"us\": %.2f}", s.c_str(), us);
+                rep += buf;
+                if (us < best.us) {
+                    c.us = us;
+                    best = c;
+                }
+            }
+        }
+    }
     cleanup();
     rep += "]}";
     if (best.us >= 1e30) return set_error(ESTL_ERR_UNSUPPORTED, "no candidate plan ran for n=%ld", n);
