@@ -40,6 +40,5 @@ struct UArgs {
 hipError_t launch_ubatch(int dir, const UArgs &a, long blocks, hipStream_t st);
 long ubatch_resident_blocks();
 int ubatch_small_rows_per_item(int log2n);
-int ubatch_split_log2(int log2n);
 
 }  // namespace estl
