@@ -27,6 +27,7 @@ FLAG_AUTOTUNE = 0x2
 CONV_CORRELATE = 0x10
 CONV_NORMALIZE_KERNEL = 0x20
 CONV_CENTER = 0x40
+CONV_FP16 = 0x80
 
 OK = 0
 ERR_INVALID = -1
@@ -97,6 +98,7 @@ def lib():
     _sig(L, "estl_conv_create", ci, [ctypes.POINTER(vp), cl, cl, ci, cu])
     _sig(L, "estl_conv_set_filter", ci, [vp, vp, cl, vp])
     _sig(L, "estl_conv_execute", ci, [vp, vp, vp, vp])
+    _sig(L, "estl_conv_execute_fp16", ci, [vp, vp, vp, vp, vp, vp])
     _sig(L, "estl_conv_workspace_bytes", ll, [vp])
     _sig(L, "estl_conv_destroy", ci, [vp])
     llp = ctypes.POINTER(ll)
@@ -104,6 +106,8 @@ def lib():
     _sig(L, "estl_server_submit_fft", ci, [vp, cl, cl, ci, vp, vp, llp])
     _sig(L, "estl_server_submit_conv", ci, [vp, cl, cl, vp, vp, cl, cu, vp, llp])
     _sig(L, "estl_server_register_filter", ci, [vp, cl, vp, cl, cu, vp, llp])
+    _sig(L, "estl_server_submit_conv_fp16", ci, [vp, cl, cl, vp, vp, vp, cl, cu, vp, llp])
+    _sig(L, "estl_server_submit_conv_filter_fp16", ci, [vp, ll, cl, vp, vp, cu, vp, llp])
     _sig(L, "estl_server_submit_conv_filter", ci, [vp, ll, cl, vp, cu, vp, llp])
     _sig(L, "estl_server_next_done", ci, [vp, llp, ci, ctypes.c_double])
     _sig(L, "estl_server_poll", ci, [vp, ll, ctypes.POINTER(ci), ctypes.POINTER(vp),
@@ -308,6 +312,11 @@ class Convolution:
         py = d_y if isinstance(d_y, int) else d_y.data_ptr()
         check(lib().estl_conv_execute(self._h, px, py, _stream_ptr(stream)))
 
+    def execute_fp16(self, d_x16, x_scale, d_y16, y_scale, stream=None) -> None:
+        """FP16-storage object: binary16 real rows with one FP32 scale per row."""
+        p = [t if isinstance(t, int) else t.data_ptr() for t in (d_x16, x_scale, d_y16, y_scale)]
+        check(lib().estl_conv_execute_fp16(self._h, p[0], p[1], p[2], p[3], _stream_ptr(stream)))
+
     def workspace_bytes(self) -> int:
         return check(lib().estl_conv_workspace_bytes(self._h))
 
@@ -374,6 +383,31 @@ class Server:
         check(lib().estl_server_submit_conv_filter(self._h, filter_id, rows, self._ptr(d_x), flags,
                                                    _stream_ptr(stream), ctypes.byref(rid)))
         return rid.value
+
+    def submit_conv_fp16(self, n: int, rows: int, d_x16, x_scale, d_h, k: int | None = None, flags: int = 0,
+                         stream=None) -> int:
+        """FP16 storage: binary16 real rows with per-row FP32 scales (see output_fp16)."""
+        k = k if k is not None else d_h.numel()
+        rid = ctypes.c_longlong()
+        check(lib().estl_server_submit_conv_fp16(self._h, n, rows, self._ptr(d_x16), self._ptr(x_scale),
+                                                 self._ptr(d_h), k, flags, _stream_ptr(stream), ctypes.byref(rid)))
+        return rid.value
+
+    def submit_conv_filter_fp16(self, filter_id: int, rows: int, d_x16, x_scale, flags: int = 0,
+                                stream=None) -> int:
+        rid = ctypes.c_longlong()
+        check(lib().estl_server_submit_conv_filter_fp16(self._h, filter_id, rows, self._ptr(d_x16),
+                                                        self._ptr(x_scale), flags, _stream_ptr(stream),
+                                                        ctypes.byref(rid)))
+        return rid.value
+
+    def output_fp16(self, rid: int, n: int, rows: int, stream=None):
+        """(binary16 rows, FP32 row scales) of a DONE FP16 request, as new tensors."""
+        import torch
+        off = (2 * n * rows + 255) // 256 * 256
+        raw = torch.empty(off + 4 * rows, dtype=torch.uint8, device="cuda")
+        check(lib().estl_server_copy_output(self._h, rid, raw.data_ptr(), raw.numel(), _stream_ptr(stream)))
+        return raw[:2 * n * rows].view(torch.float16), raw[off:off + 4 * rows].view(torch.float32)
 
     def next_done(self, max_ids: int = 256, timeout: float = 1.0) -> list:
         """Ids of requests that finished since the last call (oldest first)."""

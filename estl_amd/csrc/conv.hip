@@ -16,7 +16,8 @@
 //                   correlation, Convolution.h:125-131), inverse FFT, scale, and the
 //                   re / im parts go back to the two rows: one HBM read and one write
 //                   per value, one launch for the whole batch;
-//   * larger n runs the same pipeline as global passes with fused I/O (pass_tile.h).
+//   * larger n runs the same pipeline as global passes with fused I/O (pass_tile.h);
+//   * k_conv_rows_h, k_filter_l1, k_spec_l1, k_conv_scales: the FP16-storage variant.
 #include "fft_block.h"
 #include "internal.h"
 
@@ -93,6 +94,99 @@ k_conv_rows(const float *__restrict__ x, float *__restrict__ y, long rows, const
     }
 }
 
+// FP16 storage: the same pipeline on binary16 real rows (FP32 arithmetic).  The
+// spectrum has been divided by the filter's L1 norm, so every stored output lies in
+// [-1, 1] (|y| <= sum |h| max |x|); the caller's per-row scales carry the magnitudes.
+template <int LOG2N, int TPW, bool CORR>
+__global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT * TPW)
+k_conv_rows_h(const _Float16 *__restrict__ x, _Float16 *__restrict__ y, long rows, const float2 *__restrict__ spec,
+              const float2 *__restrict__ tw, float scale) {
+    using G = BlockGeom<LOG2N>;
+    constexpr int N = G::N, E = G::E, TPT = G::TPT;
+    constexpr int SLICE = G::uses_lds ? Lds<kLdsPad>::size(N) : 0;
+    constexpr int LDS = G::uses_lds ? TPW * SLICE : 1;
+    __shared__ float2 lds[LDS];
+    const long pairs = (rows + 1) / 2;
+    const long groups = (pairs + TPW - 1) / TPW;
+    for (long grp = blockIdx.x; grp < groups; grp += gridDim.x) {  // uniform across the workgroup
+        const int t = opaque_lane(threadIdx.x);
+        const int w = t / TPT, tl = t % TPT;
+        const long pr = grp * TPW + w;
+        const bool active = pr < pairs;
+        const long r1 = active ? 2 * pr : 0;
+        const bool two = active && r1 + 1 < rows;
+        const _Float16 *x1 = x + r1 * N + tl;
+        float2 v[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            v[i] = make_float2((float)x1[TPT * i], two ? (float)x1[N + TPT * i] : 0.0f);
+        block_fft<LOG2N, -1, kLdsPad>(v, lds + w * SLICE, tl, tw, true);
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const float2 hk = spec[tl + TPT * i];
+            v[i] = CORR ? cmulc(v[i], hk) : cmul(v[i], hk);
+        }
+        block_fft<LOG2N, +1, kLdsPad>(v, lds + w * SLICE, tl, tw, true);
+        if (active) {
+            _Float16 *y1 = y + r1 * N + tl;
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                y1[TPT * i] = (_Float16)(v[i].x * scale);
+                if (two) y1[N + TPT * i] = (_Float16)(v[i].y * scale);
+            }
+        }
+    }
+}
+
+// l1->y = sum |h_t| over the k filter taps (one workgroup, fixed summation order, so
+// the value is the same on every call); it bounds the L1 norm of the folded filter.
+__global__ void __launch_bounds__(256) k_filter_l1(const float *__restrict__ h, long k, float2 *__restrict__ l1) {
+    __shared__ float part[256];
+    float acc = 0.0f;
+    for (long i = threadIdx.x; i < k; i += 256) acc += fabsf(h[i]);
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) l1->y = part[0];
+}
+
+// spectrum /= L, L = l1->y (divided by |K^[0]| when the spectrum was normalised by the
+// kernel sum, dc); thread 0 records L in l1->x (the output scale factor)
+__global__ void k_spec_l1(float2 *__restrict__ spec, long n, float2 *__restrict__ l1, const float2 *__restrict__ dc,
+                          int normalized) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float L = l1->y;
+    if (normalized) L /= fabsf(dc->x);
+    if (!(L > 0.0f)) L = 1.0f;
+    if (i < n) {
+        const float s = 1.0f / L;
+        spec[i] = make_float2(spec[i].x * s, spec[i].y * s);
+    }
+    if (i == 0) l1->x = L;
+}
+
+__global__ void k_conv_scales(const float *__restrict__ s_in, float *__restrict__ s_out, const float2 *__restrict__ l1,
+                              long rows) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < rows) s_out[i] = s_in[i] * l1->x;
+}
+
+hipError_t launch_conv_fp16_filter(const float *h, long k, float2 *spec, long n, const float2 *dc, bool normalized,
+                                   hipStream_t st) {
+    hipLaunchKernelGGL(k_filter_l1, dim3(1), dim3(256), 0, st, h, k, spec + n);
+    hipLaunchKernelGGL(k_spec_l1, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, spec, n, spec + n, dc,
+                       normalized ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1, long rows, hipStream_t st) {
+    hipLaunchKernelGGL(k_conv_scales, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, s_in, s_out, l1, rows);
+    return hipGetLastError();
+}
+
 template <class K>
 static long conv_resident(K kernel, int threads) {
     int per_cu = 0, cus = 0, dev = 0;
@@ -116,6 +210,41 @@ static hipError_t launch_conv_rows_t(const float *x, float *y, long rows, const 
     if (blocks > 4 * resident) blocks = 4 * resident;
     hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(G::TPT * TPW), 0, st, x, y, rows, spec, tw, scale);
     return hipGetLastError();
+}
+
+template <int LOG2N, bool CORR>
+static hipError_t launch_conv_rows_h_t(const void *x, void *y, long rows, const float2 *spec, const float2 *tw,
+                                       float scale, hipStream_t st) {
+    using G = BlockGeom<LOG2N>;
+    constexpr int TPW = (G::TPT >= 256) ? 1 : 256 / G::TPT;
+    auto kernel = k_conv_rows_h<LOG2N, TPW, CORR>;
+    static long resident = 0;  // per process; one device per process on this path
+    if (!resident) resident = conv_resident(kernel, G::TPT * TPW);
+    const long pairs = (rows + 1) / 2;
+    long blocks = (pairs + TPW - 1) / TPW;
+    if (blocks > 4 * resident) blocks = 4 * resident;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(G::TPT * TPW), 0, st, static_cast<const _Float16 *>(x),
+                       static_cast<_Float16 *>(y), rows, spec, tw, scale);
+    return hipGetLastError();
+}
+
+template <bool CORR>
+static hipError_t launch_conv_rows_h_c(int log2n, const void *x, void *y, long rows, const float2 *spec,
+                                       const float2 *tw, float scale, hipStream_t st) {
+    switch (log2n) {
+#define CONV_H_CASE(L) \
+    case L: return launch_conv_rows_h_t<L, CORR>(x, y, rows, spec, tw, scale, st);
+        CONV_H_CASE(1) CONV_H_CASE(2) CONV_H_CASE(3) CONV_H_CASE(4) CONV_H_CASE(5) CONV_H_CASE(6) CONV_H_CASE(7)
+        CONV_H_CASE(8) CONV_H_CASE(9) CONV_H_CASE(10) CONV_H_CASE(11) CONV_H_CASE(12) CONV_H_CASE(13)
+#undef CONV_H_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_conv_rows_fp16(int log2n, bool corr, const void *x, void *y, long rows, const float2 *spec,
+                                 const float2 *tw, float scale, hipStream_t st) {
+    return corr ? launch_conv_rows_h_c<true>(log2n, x, y, rows, spec, tw, scale, st)
+                : launch_conv_rows_h_c<false>(log2n, x, y, rows, spec, tw, scale, st);
 }
 
 template <bool CORR>

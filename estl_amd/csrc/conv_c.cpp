@@ -27,9 +27,18 @@ hipError_t launch_conv_rows(int log2n, bool corr, const float *x, float *y, long
                             const float2 *tw, float scale, hipStream_t st);
 hipError_t launch_conv_fold(const float *h, long k, long n, long shift, float2 *hp, hipStream_t st);
 hipError_t launch_spec_normalize(float2 *spec, long n, const float2 *dc, hipStream_t st);
+hipError_t launch_conv_rows_fp16(int log2n, bool corr, const void *x, void *y, long rows, const float2 *spec,
+                                 const float2 *tw, float scale, hipStream_t st);
+hipError_t launch_conv_fp16_filter(const float *h, long k, float2 *spec, long n, const float2 *dc, bool normalized,
+                                   hipStream_t st);
+hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1, long rows, hipStream_t st);
 
-int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags) {
+int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precision) {
     if (n < 2 || (n & (n - 1))) return set_error(ESTL_ERR_INVALID, "n=%ld is not a power of two >= 2", n);
+    if (precision != ESTL_PRECISION_FP32 && precision != ESTL_PRECISION_FP16)
+        return set_error(ESTL_ERR_UNSUPPORTED, "convolution precision %d not supported", precision);
+    const bool h = precision == ESTL_PRECISION_FP16;
+    c->precision = precision;
     if (rows < 1) return set_error(ESTL_ERR_INVALID, "rows=%ld must be >= 1", rows);
     const unsigned known = ESTL_CONV_CORRELATE | ESTL_CONV_NORMALIZE_KERNEL | ESTL_CONV_CENTER;
     if (flags & ~known) return set_error(ESTL_ERR_INVALID, "unknown convolution flags 0x%x", flags);
@@ -41,7 +50,7 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags) {
     c->log2n = 0;
     while ((1L << c->log2n) < n) ++c->log2n;
     c->scale = 1.0f / (float)n;
-    c->spec_bytes = sizeof(float2) * (size_t)n;
+    c->spec_bytes = sizeof(float2) * (size_t)(h ? n + 1 : n);
     int rc = build_plan(&c->spec_plan, n, 1, ESTL_FORWARD, ESTL_PRECISION_FP32, 0, nullptr);
     if (rc) return rc;
     if (c->log2n <= kMaxRowsLog2) {
@@ -52,12 +61,14 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags) {
     // forward passes with the default split; the inverse runs the radices in the reverse
     // order, so the last forward pass and the first inverse pass cover the same columns
     // and fuse (with the spectrum multiply) into one middle pass (pass_tile.h, MID)
+    // FP16 storage: every pass stores 1/sqrt(radix) of its result, so the forward and
+    // inverse passes together scale by exactly 1/n
     Plan f, b;
-    rc = build_plan(&f, n, c->pairs, ESTL_FORWARD, ESTL_PRECISION_FP32, 0, nullptr);
+    rc = build_plan(&f, n, c->pairs, ESTL_FORWARD, precision, 0, nullptr);
     if (rc) return rc;
     std::vector<int> rev;
     for (auto it = f.steps.rbegin(); it != f.steps.rend(); ++it) rev.push_back(it->log2b);
-    rc = build_plan(&b, n, c->pairs, ESTL_BACKWARD, ESTL_PRECISION_FP32, 0, &rev);
+    rc = build_plan(&b, n, c->pairs, ESTL_BACKWARD, precision, 0, &rev);
     if (rc) return rc;
     const size_t P = f.steps.size();
     if (P < 2 || b.steps.size() != P) return set_error(ESTL_ERR_UNSUPPORTED, "convolution of n=%ld needs >= 2 passes", n);
@@ -65,6 +76,7 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags) {
     c->fwd.assign(f.steps.begin(), f.steps.end() - 1);
     c->mid = f.steps.back();
     c->mid.args.corr = c->corr ? 1 : 0;
+    if (h) c->mid.args.scale = f.steps.back().args.scale * b.steps.front().args.scale;
     c->inv.assign(b.steps.begin() + 1, b.steps.end());
     // the first pass reads real pairs through its direct (S >= T) load path
     PassStep &f0 = c->fwd.front();
@@ -73,14 +85,19 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags) {
     f0.args.real_rows = rows;
     c->inv.back().args.io_out = 2;
     c->inv.back().args.real_rows = rows;
-    c->scratch_bytes = sizeof(float2) * (size_t)n * (size_t)c->pairs;
+    c->scratch_bytes = (h ? 4 : sizeof(float2)) * (size_t)n * (size_t)c->pairs;
     return ESTL_OK;
+}
+
+size_t conv_scratch_bytes(const ConvShape &c) {
+    const size_t spec_need = c.spec_plan.kind == Plan::PASSES ? c.spec_plan.scratch_bytes : 0;
+    return c.scratch_bytes > spec_need ? c.scratch_bytes : spec_need;
 }
 
 size_t conv_footprint(const ConvShape &c) {
     // spectrum + folded filter + dc + two scratch buffers (the spectrum FFT of a
     // multi-pass size borrows the scratch)
-    return 2 * c.spec_bytes + 256 + 2 * c.scratch_bytes;
+    return 2 * c.spec_bytes + 256 + 2 * conv_scratch_bytes(c);
 }
 
 int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k, hipStream_t st) {
@@ -97,11 +114,18 @@ int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k
         e = launch_spec_normalize(b.spec, c.n, b.dc, st);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "normalize launch: %s", hipGetErrorString(e));
     }
+    if (c.precision == ESTL_PRECISION_FP16) {
+        // divide the spectrum by L = sum |h| (/ |kernel sum| when normalised), kept at spec[n]
+        e = launch_conv_fp16_filter(h, k, b.spec, c.n, b.dc, (c.flags & ESTL_CONV_NORMALIZE_KERNEL) != 0, st);
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "FP16 filter scaling: %s", hipGetErrorString(e));
+    }
     return ESTL_OK;
 }
 
 int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y, hipStream_t st) {
     if (!x || !y) return set_error(ESTL_ERR_INVALID, "null buffer");
+    if (c.precision != ESTL_PRECISION_FP32)
+        return set_error(ESTL_ERR_INVALID, "an FP16-storage convolution runs through estl_conv_execute_fp16");
     if (c.fused) {
         hipError_t e = launch_conv_rows(c.log2n, c.corr, x, y, c.rows, b.spec, c.tw_rows, c.scale, st);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "k_conv_rows launch: %s", hipGetErrorString(e));
@@ -133,6 +157,41 @@ int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y,
     return ESTL_OK;
 }
 
+int conv_run_fp16(const ConvShape &c, const ConvBuffers &b, const void *x16, const float *s_in, void *y16,
+                  float *s_out, hipStream_t st) {
+    if (!x16 || !y16 || !s_in || !s_out) return set_error(ESTL_ERR_INVALID, "null buffer");
+    if (c.precision != ESTL_PRECISION_FP16) return set_error(ESTL_ERR_INVALID, "not an FP16-storage convolution");
+    hipError_t e;
+    if (c.fused) {
+        e = launch_conv_rows_fp16(c.log2n, c.corr, x16, y16, c.rows, b.spec, c.tw_rows, c.scale, st);
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "k_conv_rows_h launch: %s", hipGetErrorString(e));
+    } else {
+        int cur = 0;
+        const void *src = b.scratch[1];  // not read by the first pass (it reads x16)
+        auto run = [&](const PassStep &s, int dir) -> int {
+            PassArgs a = s.args;
+            a.rin16 = x16;
+            a.rout16 = y16;
+            a.spectrum = b.spec;
+            void *dst = b.scratch[cur];
+            hipError_t le = dir ? launch_pass_fp16(s.log2b, s.log2t, dir, src, dst, a, st)
+                                : launch_pass_mid_fp16(s.log2b, s.log2t, src, dst, a, st);
+            if (le != hipSuccess) return set_error(ESTL_ERR_HIP, "FP16 conv pass launch: %s", hipGetErrorString(le));
+            src = dst;
+            cur ^= 1;
+            return ESTL_OK;
+        };
+        for (const auto &s : c.fwd)
+            if (int rc = run(s, ESTL_FORWARD)) return rc;
+        if (int rc = run(c.mid, 0)) return rc;
+        for (const auto &s : c.inv)
+            if (int rc = run(s, ESTL_BACKWARD)) return rc;
+    }
+    e = launch_conv_scales(s_in, s_out, b.spec + c.n, c.rows, st);
+    if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "output scales: %s", hipGetErrorString(e));
+    return ESTL_OK;
+}
+
 // the public object: a shape plus buffers of its own
 struct Conv {
     ConvShape shape;
@@ -158,10 +217,8 @@ extern "C" {
 int estl_conv_create(estl_conv_t *conv, long n, long rows, int precision, unsigned flags) {
     if (!conv) return set_error(ESTL_ERR_INVALID, "handle pointer is null");
     *conv = nullptr;
-    if (precision != ESTL_PRECISION_FP32)
-        return set_error(ESTL_ERR_UNSUPPORTED, "convolution precision %d not supported", precision);
     Conv *c = new Conv();
-    int rc = conv_shape_init(&c->shape, n, rows, flags);
+    int rc = conv_shape_init(&c->shape, n, rows, flags, precision);
     if (rc) {
         conv_free(c);
         return rc;
@@ -169,8 +226,7 @@ int estl_conv_create(estl_conv_t *conv, long n, long rows, int precision, unsign
     const ConvShape &s = c->shape;
     bool ok = hipMalloc(&c->buf.spec, s.spec_bytes) == hipSuccess && hipMalloc(&c->buf.hp, s.spec_bytes) == hipSuccess &&
               hipMalloc(&c->buf.dc, sizeof(float2)) == hipSuccess;
-    // the multi-pass spectrum FFT needs n values of scratch even when rows need none
-    const size_t sb = s.scratch_bytes ? s.scratch_bytes : (s.spec_plan.kind == Plan::PASSES ? s.spec_bytes : 0);
+    const size_t sb = conv_scratch_bytes(s);
     for (auto &p : c->buf.scratch)
         if (ok && sb) ok = hipMalloc(&p, sb) == hipSuccess;
     if (!ok) {
@@ -194,6 +250,14 @@ int estl_conv_execute(estl_conv_t conv, const float *d_x, float *d_y, void *stre
     Conv *c = reinterpret_cast<Conv *>(conv);
     if (!c->have_filter) return set_error(ESTL_ERR_INVALID, "estl_conv_set_filter has not been called");
     return conv_run(c->shape, c->buf, d_x, d_y, static_cast<hipStream_t>(stream));
+}
+
+int estl_conv_execute_fp16(estl_conv_t conv, const void *d_x16, const float *d_x_scale, void *d_y16,
+                           float *d_y_scale, void *stream) {
+    if (!conv) return set_error(ESTL_ERR_INVALID, "handle is null");
+    Conv *c = reinterpret_cast<Conv *>(conv);
+    if (!c->have_filter) return set_error(ESTL_ERR_INVALID, "estl_conv_set_filter has not been called");
+    return conv_run_fp16(c->shape, c->buf, d_x16, d_x_scale, d_y16, d_y_scale, static_cast<hipStream_t>(stream));
 }
 
 long long estl_conv_workspace_bytes(estl_conv_t conv) {

@@ -184,7 +184,7 @@ hipError_t launch_rows_fp16(int log2n, int dir, const void *in16, void *out16, l
 }
 
 // ------------------------------------------------------------------ global passes
-template <int LOG2B, int DIR, int LOG2T>
+template <int LOG2B, int DIR, int LOG2T, bool MID>
 __global__ void __launch_bounds__(BlockGeom<LOG2B>::TPT << LOG2T)
 k_fft_pass_h(const f16x2 *__restrict__ in, f16x2 *__restrict__ out, PassArgs a) {
     __shared__ float2 lds[PassTile<LOG2B, LOG2T>::lds_size];
@@ -193,14 +193,14 @@ k_fft_pass_h(const f16x2 *__restrict__ in, f16x2 *__restrict__ out, PassArgs a) 
         const long nb = gridDim.x;
         if ((nb & 7) == 0) tile = (tile & 7) * (nb >> 3) + (tile >> 3);
     }
-    pass_tile<LOG2B, DIR, LOG2T, false, f16x2>(in, out, a, tile, lds, false);
+    pass_tile<LOG2B, DIR, LOG2T, MID, f16x2>(in, out, a, tile, lds, false);
 }
 
 // FP16 passes: radices 2^4 .. 2^9 (every split the default planner makes), tiles of at
 // most 2^13 values
 constexpr int kMaxPassLog2Fp16 = 9;
 
-template <int LOG2B, int DIR, int LOG2T>
+template <int LOG2B, int DIR, int LOG2T, bool MID>
 static hipError_t launch_pass_h_t(const f16x2 *in, f16x2 *out, const PassArgs &a, hipStream_t st) {
     if constexpr (LOG2B + LOG2T > 13) {
         return hipErrorInvalidValue;
@@ -208,36 +208,36 @@ static hipError_t launch_pass_h_t(const f16x2 *in, f16x2 *out, const PassArgs &a
         const int lcpr = a.log2n - LOG2B;
         if (lcpr < LOG2T) return hipErrorInvalidValue;
         const long blocks = a.batch << (lcpr - LOG2T);
-        hipLaunchKernelGGL((k_fft_pass_h<LOG2B, DIR, LOG2T>), dim3((unsigned)blocks), dim3(BlockGeom<LOG2B>::TPT << LOG2T),
+        hipLaunchKernelGGL((k_fft_pass_h<LOG2B, DIR, LOG2T, MID>), dim3((unsigned)blocks), dim3(BlockGeom<LOG2B>::TPT << LOG2T),
                            0, st, in, out, a);
         return hipGetLastError();
     }
 }
 
-template <int LOG2B, int DIR>
+template <int LOG2B, int DIR, bool MID>
 static hipError_t launch_pass_h_b(int log2t, const f16x2 *in, f16x2 *out, const PassArgs &a, hipStream_t st) {
     switch (log2t) {
-        case 0: return launch_pass_h_t<LOG2B, DIR, 0>(in, out, a, st);
-        case 1: return launch_pass_h_t<LOG2B, DIR, 1>(in, out, a, st);
-        case 2: return launch_pass_h_t<LOG2B, DIR, 2>(in, out, a, st);
-        case 3: return launch_pass_h_t<LOG2B, DIR, 3>(in, out, a, st);
-        case 4: return launch_pass_h_t<LOG2B, DIR, 4>(in, out, a, st);
-        case 5: return launch_pass_h_t<LOG2B, DIR, 5>(in, out, a, st);
-        case 6: return launch_pass_h_t<LOG2B, DIR, 6>(in, out, a, st);
+        case 0: return launch_pass_h_t<LOG2B, DIR, 0, MID>(in, out, a, st);
+        case 1: return launch_pass_h_t<LOG2B, DIR, 1, MID>(in, out, a, st);
+        case 2: return launch_pass_h_t<LOG2B, DIR, 2, MID>(in, out, a, st);
+        case 3: return launch_pass_h_t<LOG2B, DIR, 3, MID>(in, out, a, st);
+        case 4: return launch_pass_h_t<LOG2B, DIR, 4, MID>(in, out, a, st);
+        case 5: return launch_pass_h_t<LOG2B, DIR, 5, MID>(in, out, a, st);
+        case 6: return launch_pass_h_t<LOG2B, DIR, 6, MID>(in, out, a, st);
         default: return hipErrorInvalidValue;
     }
 }
 
-template <int DIR>
+template <int DIR, bool MID = false>
 static hipError_t launch_pass_h_d(int log2b, int log2t, const f16x2 *in, f16x2 *out, const PassArgs &a,
                                   hipStream_t st) {
     switch (log2b) {
-        case 4: return launch_pass_h_b<4, DIR>(log2t, in, out, a, st);
-        case 5: return launch_pass_h_b<5, DIR>(log2t, in, out, a, st);
-        case 6: return launch_pass_h_b<6, DIR>(log2t, in, out, a, st);
-        case 7: return launch_pass_h_b<7, DIR>(log2t, in, out, a, st);
-        case 8: return launch_pass_h_b<8, DIR>(log2t, in, out, a, st);
-        case 9: return launch_pass_h_b<9, DIR>(log2t, in, out, a, st);
+        case 4: return launch_pass_h_b<4, DIR, MID>(log2t, in, out, a, st);
+        case 5: return launch_pass_h_b<5, DIR, MID>(log2t, in, out, a, st);
+        case 6: return launch_pass_h_b<6, DIR, MID>(log2t, in, out, a, st);
+        case 7: return launch_pass_h_b<7, DIR, MID>(log2t, in, out, a, st);
+        case 8: return launch_pass_h_b<8, DIR, MID>(log2t, in, out, a, st);
+        case 9: return launch_pass_h_b<9, DIR, MID>(log2t, in, out, a, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -247,6 +247,14 @@ hipError_t launch_pass_fp16(int log2b, int log2t, int dir, const void *in16, voi
     const f16x2 *in = static_cast<const f16x2 *>(in16);
     f16x2 *out = static_cast<f16x2 *>(out16);
     return dir < 0 ? launch_pass_h_d<-1>(log2b, log2t, in, out, a, st) : launch_pass_h_d<+1>(log2b, log2t, in, out, a, st);
+}
+
+// the FP16 convolution's middle pass: last forward pass, spectrum multiply, first
+// inverse pass (pass_tile.h, MID)
+hipError_t launch_pass_mid_fp16(int log2b, int log2t, const void *in16, void *out16, const PassArgs &a,
+                                hipStream_t st) {
+    return launch_pass_h_d<-1, true>(log2b, log2t, static_cast<const f16x2 *>(in16), static_cast<f16x2 *>(out16), a,
+                                     st);
 }
 
 int max_pass_log2_fp16() { return kMaxPassLog2Fp16; }

@@ -27,6 +27,8 @@ struct PassArgs {
     long real_rows;          // real rows behind the pairs (the last pair may be single)
     const float *rin;
     float *rout;
+    const void *rin16;       // the same, binary16 reals (FP16-storage passes)
+    void *rout16;
     const float2 *spectrum;
 };
 
@@ -42,6 +44,8 @@ hipError_t launch_rows_fp16(int log2n, int dir, const void *in16, void *out16, l
                             hipStream_t st);
 hipError_t launch_pass_fp16(int log2b, int log2t, int dir, const void *in16, void *out16, const PassArgs &a,
                             hipStream_t st);
+hipError_t launch_pass_mid_fp16(int log2b, int log2t, const void *in16, void *out16, const PassArgs &a,
+                                hipStream_t st);
 int max_pass_log2_fp16();
 // forward last pass + spectrum multiply + inverse first pass of a convolution
 hipError_t launch_pass_mid(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st);

@@ -68,16 +68,26 @@ __device__ __forceinline__ void pass_tile_load(const ST *__restrict__ in, const 
     constexpr int E = G::E, TPT = G::TPT, NT = TPT << LOG2T;
     const TilePos<LOG2B, LOG2T> P(a, tile);
     const int ls = a.log2S;
-    if (F32 && a.io_in == 1) {
-        // real pair: rows 2 row and 2 row + 1 of rin (the second absent past the end);
-        // only the first pass reads pairs and its plans keep S >= T
-        const float *x1 = a.rin + ((2 * P.row) << a.log2n);
+    if (a.io_in == 1) {
+        // real pair: rows 2 row and 2 row + 1 of rin (the second absent past the end),
+        // FP32 or, with FP16 storage, binary16 reals; only the first pass reads pairs
+        // and its plans keep S >= T
         const bool two = 2 * P.row + 1 < a.real_rows;
         const int base = (P.g << (ls + LOG2B)) + P.z;
+        if constexpr (F32) {
+            const float *x1 = a.rin + ((2 * P.row) << a.log2n);
 #pragma unroll
-        for (int i = 0; i < E; ++i) {
-            const int o = base + ((P.tl + TPT * i) << ls);
-            v[i] = make_float2(ld_stream_f(x1 + o), two ? ld_stream_f(x1 + (1L << a.log2n) + o) : 0.0f);
+            for (int i = 0; i < E; ++i) {
+                const int o = base + ((P.tl + TPT * i) << ls);
+                v[i] = make_float2(ld_stream_f(x1 + o), two ? ld_stream_f(x1 + (1L << a.log2n) + o) : 0.0f);
+            }
+        } else {
+            const _Float16 *x1 = static_cast<const _Float16 *>(a.rin16) + ((2 * P.row) << a.log2n);
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                const int o = base + ((P.tl + TPT * i) << ls);
+                v[i] = make_float2((float)x1[o], two ? (float)x1[(1L << a.log2n) + o] : 0.0f);
+            }
         }
     } else if (ls >= LOG2T) {
         const ST *p = in + (P.row << a.log2n) + ((P.g << (ls + LOG2B)) + P.z);
@@ -104,7 +114,7 @@ __device__ __forceinline__ void pass_tile_load(const ST *__restrict__ in, const 
 // saving one HBM round trip and one launch.
 //
 // ST is the storage type of `in` and `out`: float2, or f16x2 (FP16 storage, FP32
-// arithmetic; the real-pair I/O modes exist for float2 only).
+// arithmetic; its real-pair I/O modes read and write binary16 reals, rin16 / rout16).
 template <int LOG2B, int DIR, int LOG2T, bool MID = false, typename ST = float2>
 __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const PassArgs &a, long tile,
                                                  float2 (&v)[BlockGeom<LOG2B>::E], float2 *lds, bool lds_busy) {
@@ -117,7 +127,7 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
     const int lcpr = a.log2n - LOG2B;
     const int c = P.c, tl = P.tl, j = P.j;
     bool busy = lds_busy;
-    if (!(F32 && a.io_in == 1) && ls < LOG2T) {
+    if (a.io_in != 1 && ls < LOG2T) {
         // value o of the contiguous block belongs to column (o / (S B)) S + o mod S,
         // element (o mod S B) / S
         const int sbmask = (1 << (ls + LOG2B)) - 1, smask = (1 << ls) - 1;
@@ -160,14 +170,25 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
         }
     }
     block_fft<LOG2B, DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, busy);
-    if (F32 && a.io_out == 2) {
-        float *y1 = a.rout + ((2 * P.row) << a.log2n) + j;
+    if (a.io_out == 2) {
+        // re / im to real rows 2 row and 2 row + 1 of rout (FP32 or binary16)
         const bool two = 2 * P.row + 1 < a.real_rows;
+        if constexpr (F32) {
+            float *y1 = a.rout + ((2 * P.row) << a.log2n) + j;
 #pragma unroll
-        for (int i = 0; i < E; ++i) {
-            const int o = (tl + TPT * i) << lcpr;
-            st_stream_f(y1 + o, v[i].x * a.scale);
-            if (two) st_stream_f(y1 + (1L << a.log2n) + o, v[i].y * a.scale);
+            for (int i = 0; i < E; ++i) {
+                const int o = (tl + TPT * i) << lcpr;
+                st_stream_f(y1 + o, v[i].x * a.scale);
+                if (two) st_stream_f(y1 + (1L << a.log2n) + o, v[i].y * a.scale);
+            }
+        } else {
+            _Float16 *y1 = static_cast<_Float16 *>(a.rout16) + ((2 * P.row) << a.log2n) + j;
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                const int o = (tl + TPT * i) << lcpr;
+                y1[o] = (_Float16)(v[i].x * a.scale);
+                if (two) y1[(1L << a.log2n) + o] = (_Float16)(v[i].y * a.scale);
+            }
         }
         return;
     }

@@ -65,6 +65,7 @@ int tune_plan(long n, long batch, int dir, bool *rows, std::vector<int> *split, 
 
 // ---- batched circular convolution: immutable shape + per-execution buffers
 struct ConvShape {
+    int precision = 0;  // ESTL_PRECISION_FP32 or _FP16 (binary16 rows, per-row scales)
     int log2n = 0;
     long n = 0, rows = 0, pairs = 0;
     unsigned flags = 0;
@@ -75,15 +76,22 @@ struct ConvShape {
     std::vector<PassStep> fwd, inv;  // multi-pass pipeline (n > 2^13): fwd, mid, inv
     PassStep mid{};                  // last forward + multiply + first inverse pass
     size_t scratch_bytes = 0;        // bytes of each of the two scratch buffers
-    size_t spec_bytes = 0;           // bytes of the spectrum (and of the folded filter)
+    size_t spec_bytes = 0;           // bytes of the spectrum (FP16: n + 1 values, the
+                                     // last holding the filter's L1 bound)
 };
 struct ConvBuffers {
     float2 *spec = nullptr, *hp = nullptr, *dc = nullptr;
     float2 *scratch[2] = {nullptr, nullptr};
 };
-int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags);
+int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precision = 0);
 size_t conv_footprint(const ConvShape &c);  // bytes of ConvBuffers
+// bytes of each of the two scratch buffers: the pipeline's, or more if the filter's
+// (FP32) spectrum transform, which borrows scratch[0], needs more
+size_t conv_scratch_bytes(const ConvShape &c);
 int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k, hipStream_t st);
 int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y, hipStream_t st);
+// FP16 storage: x16 / y16 binary16 rows, y = (stored y) * s_out[row]
+int conv_run_fp16(const ConvShape &c, const ConvBuffers &b, const void *x16, const float *s_in, void *y16,
+                  float *s_out, hipStream_t st);
 
 }  // namespace estl

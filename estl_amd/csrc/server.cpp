@@ -119,6 +119,7 @@ struct Request {
     unsigned flags = 0;
     const void *in = nullptr;
     const float *h = nullptr;
+    const float *xscale = nullptr;  // FP16 storage (ESTL_CONV_FP16): per-row input scales
     long long filter = 0;  // KIND_CONV_REG: the registration it uses
     int users = 0;         // KIND_FILTER: requests queued or running on it
     // arena blocks: [0] output (the spectrum for KIND_FILTER), the rest workspace
@@ -197,6 +198,8 @@ class Server {
             if (it->second.state == FAILED) return set_error(ESTL_ERR_INVALID, "filter %lld failed", r.filter);
             filter = &it->second;
             r.n = filter->n;
+            if ((filter->flags & ESTL_CONV_FP16) != (r.flags & ESTL_CONV_FP16))
+                return set_error(ESTL_ERR_INVALID, "filter %lld and the request differ in FP16 storage", r.filter);
         }
         // the server's streams wait for the work already queued on the submitter's stream
         int rc = get_event(&r.ev_in);
@@ -372,13 +375,14 @@ class Server {
         auto it = conv_shapes_.find(key);
         if (it != conv_shapes_.end()) return it->second.get();
         std::unique_ptr<ConvShape> c(new ConvShape());
-        if (conv_shape_init(c.get(), n, rows, flags)) return nullptr;
+        const int prec = (flags & ESTL_CONV_FP16) ? ESTL_PRECISION_FP16 : ESTL_PRECISION_FP32;
+        if (conv_shape_init(c.get(), n, rows, flags & ~(unsigned)ESTL_CONV_FP16, prec)) return nullptr;
         ConvShape *raw = c.get();
         conv_shapes_[key] = std::move(c);
         return raw;
     }
     static size_t spec_scratch(const ConvShape &c) {
-        return c.spec_plan.kind == Plan::PASSES ? c.spec_bytes : 0;
+        return c.spec_plan.kind == Plan::PASSES ? c.spec_plan.scratch_bytes : 0;
     }
 
     // sizes of the arena blocks a request needs: output first
@@ -403,18 +407,23 @@ class Server {
                 ConvShape *c = conv_shape(r.n, r.rows, r.flags);
                 if (!c) return fail();
                 r.shape = c;
-                sizes.push_back(sizeof(float) * (size_t)r.n * (size_t)r.rows);
-                if (c->scratch_bytes) {
-                    sizes.push_back(c->scratch_bytes);
-                    sizes.push_back(c->scratch_bytes);
+                if (r.flags & ESTL_CONV_FP16)  // binary16 rows, then (256-aligned) FP32 row scales
+                    sizes.push_back(fp16_out_scales_offset(r.n, r.rows) + sizeof(float) * (size_t)r.rows);
+                else
+                    sizes.push_back(sizeof(float) * (size_t)r.n * (size_t)r.rows);
+                // two scratch buffers: the pipeline's (multi-pass sizes) and, with an
+                // inline filter, the spectrum transform's (it borrows the first)
+                const size_t sb = r.kind == KIND_CONV ? conv_scratch_bytes(*c) : c->scratch_bytes;
+                if (sb) {
+                    sizes.push_back(sb);
+                    sizes.push_back(sb);
                 }
                 if (r.kind == KIND_CONV) {
                     sizes.push_back(c->spec_bytes);
                     sizes.push_back(c->spec_bytes);
                     sizes.push_back(256);
-                    if (!c->scratch_bytes && spec_scratch(*c)) sizes.push_back(spec_scratch(*c));
                 }
-                r.alg_bytes = 8.0 * r.n * r.rows + (r.kind == KIND_CONV ? 4.0 * r.k : 0.0);
+                r.alg_bytes = ((r.flags & ESTL_CONV_FP16) ? 4.0 : 8.0) * r.n * r.rows + (r.kind == KIND_CONV ? 4.0 * r.k : 0.0);
                 return true;
             }
             case KIND_FILTER: {
@@ -431,6 +440,10 @@ class Server {
         }
         why = "unknown request kind";
         return false;
+    }
+
+    static size_t fp16_out_scales_offset(long n, long rows) {
+        return (2 * (size_t)n * (size_t)rows + 255) & ~size_t(255);
     }
 
     // launches of one admitted request on stream st (called without the lock)
@@ -453,7 +466,8 @@ class Server {
         ConvShape *c = r.shape;
         ConvBuffers b;
         size_t next = 1;
-        if (c->scratch_bytes) {
+        const size_t sb = r.kind == KIND_CONV ? conv_scratch_bytes(*c) : c->scratch_bytes;
+        if (sb) {
             b.scratch[0] = ptr(next++);
             b.scratch[1] = ptr(next++);
         }
@@ -461,11 +475,15 @@ class Server {
             b.spec = ptr(next++);
             b.hp = ptr(next++);
             b.dc = ptr(next++);
-            if (!b.scratch[0] && next < r.blocks.size()) b.scratch[0] = ptr(next++);
             int rc = conv_filter(*c, b, r.h, r.k, st);
             if (rc) return rc;
         } else {
             b.spec = const_cast<float2 *>(r.spectrum);
+        }
+        if (r.flags & ESTL_CONV_FP16) {
+            char *o = static_cast<char *>(r.out);
+            return conv_run_fp16(*c, b, r.in, r.xscale, o,
+                                 reinterpret_cast<float *>(o + fp16_out_scales_offset(r.n, r.rows)), st);
         }
         return conv_run(*c, b, static_cast<const float *>(r.in), static_cast<float *>(r.out), st);
     }
@@ -714,6 +732,39 @@ int estl_server_submit_conv(estl_server_t srv, long n, long rows, const float *d
     r.flags = conv_flags;
     r.in = d_x;
     r.h = d_h;
+    return reinterpret_cast<Server *>(srv)->submit(std::move(r), static_cast<hipStream_t>(input_stream), request_id);
+}
+
+int estl_server_submit_conv_fp16(estl_server_t srv, long n, long rows, const void *d_x16, const float *d_x_scale,
+                                 const float *d_h, long k, unsigned conv_flags, void *input_stream,
+                                 long long *request_id) {
+    if (!srv || !request_id || !d_x16 || !d_x_scale || !d_h) return set_error(ESTL_ERR_INVALID, "null argument");
+    if (k < 1) return set_error(ESTL_ERR_INVALID, "filter length %ld < 1", k);
+    Request r;
+    r.kind = KIND_CONV;
+    r.n = n;
+    r.rows = rows;
+    r.k = k;
+    r.flags = conv_flags | ESTL_CONV_FP16;
+    r.in = d_x16;
+    r.xscale = d_x_scale;
+    r.h = d_h;
+    return reinterpret_cast<Server *>(srv)->submit(std::move(r), static_cast<hipStream_t>(input_stream), request_id);
+}
+
+int estl_server_submit_conv_filter_fp16(estl_server_t srv, long long filter_id, long rows, const void *d_x16,
+                                        const float *d_x_scale, unsigned conv_flags, void *input_stream,
+                                        long long *request_id) {
+    if (!srv || !request_id || !d_x16 || !d_x_scale) return set_error(ESTL_ERR_INVALID, "null argument");
+    if (conv_flags & ~(unsigned)ESTL_CONV_CORRELATE)
+        return set_error(ESTL_ERR_INVALID, "only ESTL_CONV_CORRELATE is chosen per request (0x%x)", conv_flags);
+    Request r;
+    r.kind = KIND_CONV_REG;
+    r.filter = filter_id;
+    r.rows = rows;
+    r.flags = conv_flags | ESTL_CONV_FP16;
+    r.in = d_x16;
+    r.xscale = d_x_scale;
     return reinterpret_cast<Server *>(srv)->submit(std::move(r), static_cast<hipStream_t>(input_stream), request_id);
 }
 

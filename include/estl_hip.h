@@ -51,6 +51,7 @@ extern "C" {
 #define ESTL_CONV_CORRELATE 0x10u        /* multiply by conj(H): correlation, Convolution.h:125-131 */
 #define ESTL_CONV_NORMALIZE_KERNEL 0x20u /* divide by the kernel sum K^[0], Convolution.h:141 */
 #define ESTL_CONV_CENTER 0x40u           /* rotate the filter by (K+1)/2, Convolution.h:113-120 */
+#define ESTL_CONV_FP16 0x80u             /* server: binary16 rows with per-row scales         */
 
 typedef struct estl_plan_s *estl_plan_t;
 typedef struct estl_ubatch_s *estl_ubatch_t;
@@ -216,6 +217,15 @@ int estl_conv_set_filter(estl_conv_t conv, const float *d_h, long k, void *strea
 /* y = x (*) h for all rows (device buffers of rows * n floats; asynchronous). */
 int estl_conv_execute(estl_conv_t conv, const float *d_x, float *d_y, void *stream);
 
+/* FP16-storage convolution (object created with ESTL_PRECISION_FP16; no counterpart in
+ * the reference, which stores FP32 only): d_x16 / d_y16 are rows * n binary16 reals,
+ * true x = stored * d_x_scale[row] (estl_pack_fp16 with n/2 complex values per row
+ * packs real rows this way); arithmetic is FP32 with 1/sqrt(radix) storage scaling per
+ * pass, and the filter spectrum is divided by L = sum |h|, so every stored output is
+ * in [-1, 1] and d_y_scale[row] = d_x_scale[row] * L. */
+int estl_conv_execute_fp16(estl_conv_t conv, const void *d_x16, const float *d_x_scale, void *d_y16,
+                           float *d_y_scale, void *stream);
+
 /* Device bytes the object holds (scratch, spectrum). */
 long long estl_conv_workspace_bytes(estl_conv_t conv);
 
@@ -260,9 +270,20 @@ int estl_server_register_filter(estl_server_t srv, long n, const float *d_h, lon
                                 void *input_stream, long long *filter_id);
 
 /* Convolution of `rows` real rows of the filter's n with a registered filter;
- * conv_flags may hold ESTL_CONV_CORRELATE only. */
+ * conv_flags may hold ESTL_CONV_CORRELATE only (the filter must not be FP16). */
 int estl_server_submit_conv_filter(estl_server_t srv, long long filter_id, long rows, const float *d_x,
                                    unsigned conv_flags, void *input_stream, long long *request_id);
+
+/* FP16 storage (see estl_conv_execute_fp16): d_x16 holds rows * n binary16 reals with
+ * per-row scales d_x_scale; the output (in the arena) is rows * n binary16 reals followed,
+ * at the next 256-byte boundary, by rows FP32 output scales.  A registered filter used
+ * by FP16 requests is registered with ESTL_CONV_FP16 in its flags. */
+int estl_server_submit_conv_fp16(estl_server_t srv, long n, long rows, const void *d_x16, const float *d_x_scale,
+                                 const float *d_h, long k, unsigned conv_flags, void *input_stream,
+                                 long long *request_id);
+int estl_server_submit_conv_filter_fp16(estl_server_t srv, long long filter_id, long rows, const void *d_x16,
+                                        const float *d_x_scale, unsigned conv_flags, void *input_stream,
+                                        long long *request_id);
 
 /* State (ESTL_REQ_*), the output's device address once DONE, latency submit->done. */
 int estl_server_poll(estl_server_t srv, long long request_id, int *state, void **d_out, double *latency_s);

@@ -117,3 +117,90 @@ def test_filter_reuse_and_odd_rows(gpu):
     c.execute(x, y2)
     torch.cuda.synchronize()
     assert torch.equal(y2, ys[-1])
+
+
+# ---------------------------------------------------------------- FP16 storage
+U16 = 2.0 ** -11
+
+
+def conv_fp16(torch, gpu, x, h, flags=0):
+    """x: (rows, n) float32 (values of any magnitude); returns (y float64, x as
+    dequantised from its binary16 form, launches)."""
+    rows, n = x.shape
+    c = estl_amd.Convolution(n, rows, flags, precision=estl_amd.FP16)
+    dx = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(gpu)
+    x16 = torch.empty(rows * n, dtype=torch.float16, device=gpu)
+    s_in = torch.empty(rows, dtype=torch.float32, device=gpu)
+    estl_amd.pack_fp16(dx, x16, s_in, n // 2, rows)        # real rows as n/2 complex values
+    dh = torch.from_numpy(np.ascontiguousarray(h, dtype=np.float32)).to(gpu)
+    c.set_filter(dh)
+    y16 = torch.empty_like(x16)
+    s_out = torch.empty_like(s_in)
+    c.execute_fp16(x16, s_in, y16, s_out)
+    torch.cuda.synchronize()
+    y = (y16.float().view(rows, n) * s_out[:, None]).cpu().numpy().astype(np.float64)
+    xq = (x16.float().view(rows, n) * s_in[:, None]).cpu().numpy().astype(np.float64)
+    assert float(y16.float().abs().max()) <= 1.0
+    return y, xq
+
+
+def fp16_tol(n):
+    # one binary16 store per launch: 1 (n <= 2^13) or 2P - 1 for a P-pass split
+    P = 0 if n <= 2 ** 13 else max(2, (int(math.log2(n)) + 8) // 9)
+    stores = 1 if P == 0 else 2 * P - 1
+    return U16 * math.sqrt(stores + 1)
+
+
+@pytest.mark.parametrize("n,rows,k,flags", [
+    (16, 1, 3, 0), (1024, 5, 16, 0), (4096, 3, 300, estl_amd.CONV_CORRELATE), (8192, 4, 8192, 0),
+    (2 ** 14, 3, 512, 0), (2 ** 16, 2, 1000, estl_amd.CONV_CORRELATE), (2 ** 18, 1, 64, 0),
+    (2 ** 15, 5, 33, estl_amd.CONV_CENTER), (4096, 2, 33, estl_amd.CONV_NORMALIZE_KERNEL),
+    (2 ** 14, 2, 33, estl_amd.CONV_NORMALIZE_KERNEL),
+])
+def test_conv_fp16_matches_direct_sum(gpu, n, rows, k, flags):
+    import torch
+    x = oracle.fill_uniform(rows * n, 30).astype(np.float64).reshape(rows, n)
+    x[0] *= 1e-4                     # per-row scaling: rows of very different magnitude
+    if rows > 1:
+        x[-1] *= 3e3
+    h = oracle.fill_uniform(k, 31).astype(np.float64)
+    if flags & estl_amd.CONV_NORMALIZE_KERNEL:
+        h += 1.5
+    y, xq = conv_fp16(torch, gpu, x.astype(np.float32), h.astype(np.float32), flags)
+    h32 = h.astype(np.float32).astype(np.float64)
+    # references through the oracle's direct sum over the k taps (O(n k)):
+    #   correlation y[i] = sum_j h[j] x[i + j] = (x_r (*) h)[-i] with x_r[m] = x[-m];
+    #   centring rotates the output: y[i] = (x (*) h)[i + (k + 1)/2];
+    #   normalising divides the filter by its sum
+    if flags & estl_amd.CONV_CORRELATE:
+        xr = np.roll(xq[:, ::-1], 1, axis=1)
+        ref = np.roll(oracle.conv_circular(xr, h32)[:, ::-1], 1, axis=1)
+    elif flags & estl_amd.CONV_CENTER:
+        ref = np.roll(oracle.conv_circular(xq, h32), -((k + 1) // 2), axis=1)
+    elif flags & estl_amd.CONV_NORMALIZE_KERNEL:
+        ref = oracle.conv_circular(xq, h32 / h32.sum())
+    else:
+        ref = oracle.conv_circular(xq, h32)
+    for r in range(rows):
+        e = oracle.rel_l2(y[r], ref[r])
+        assert e <= fp16_tol(n), (n, r, e, fp16_tol(n))
+
+
+def test_conv_fp16_output_scale_is_input_scale_times_l1(gpu):
+    import torch
+    n, rows, k = 4096, 2, 100
+    x = oracle.fill_uniform(rows * n, 40).astype(np.float32).reshape(rows, n)
+    h = oracle.fill_uniform(k, 41).astype(np.float32)
+    c = estl_amd.Convolution(n, rows, 0, precision=estl_amd.FP16)
+    dx = torch.from_numpy(x).to(gpu)
+    x16 = torch.empty(rows * n, dtype=torch.float16, device=gpu)
+    s_in = torch.empty(rows, dtype=torch.float32, device=gpu)
+    estl_amd.pack_fp16(dx, x16, s_in, n // 2, rows)
+    c.set_filter(torch.from_numpy(h).to(gpu))
+    y16, s_out = torch.empty_like(x16), torch.empty_like(s_in)
+    c.execute_fp16(x16, s_in, y16, s_out)
+    torch.cuda.synchronize()
+    l1 = float(np.abs(h).astype(np.float32).sum(dtype=np.float32))
+    np.testing.assert_allclose(s_out.cpu().numpy(), s_in.cpu().numpy() * l1, rtol=1e-6)
+    with pytest.raises(estl_amd.EstlError):
+        c.execute(dx, dx)            # an FP16 object has no FP32 entry point

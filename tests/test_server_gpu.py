@@ -235,3 +235,44 @@ def test_registered_filters_and_completion_queue(gpu):
     with pytest.raises(estl_amd.EstlError):
         srv.register_filter(n, h, k, estl_amd.CONV_CORRELATE)
     srv.close()
+
+
+def test_fp16_requests_equal_fp16_object_bitwise(gpu):
+    import torch
+    srv = estl_amd.Server(1 << 30, streams=3)
+    cases = [(2 ** 12, 7, 100, 0), (2 ** 16, 3, 3000, estl_amd.CONV_CORRELATE), (2 ** 20, 1, 64, 0)]
+    refs, ids, keep = [], [], []
+    for i, (n, rows, k, fl) in enumerate(cases):
+        x = torch.empty(rows * n, dtype=torch.float32, device=gpu)
+        estl_amd.fill_uniform(x, 80 + i)
+        x16 = torch.empty(rows * n, dtype=torch.float16, device=gpu)
+        s_in = torch.empty(rows, dtype=torch.float32, device=gpu)
+        estl_amd.pack_fp16(x, x16, s_in, n // 2, rows)
+        h = torch.empty(k, dtype=torch.float32, device=gpu)
+        estl_amd.fill_uniform(h, 90 + i)
+        c = estl_amd.Convolution(n, rows, fl, precision=estl_amd.FP16)
+        c.set_filter(h)
+        y16, s_out = torch.empty_like(x16), torch.empty_like(s_in)
+        c.execute_fp16(x16, s_in, y16, s_out)
+        torch.cuda.synchronize()
+        refs.append((y16.clone(), s_out.clone()))
+        keep.append((x, x16, s_in, h))   # inputs must outlive their requests
+        # inline filter, and a registered FP16 filter, beside an FP32 request
+        ids.append((i, "inline", srv.submit_conv_fp16(n, rows, x16, s_in, h, k, fl)))
+        f = srv.register_filter(n, h, k, estl_amd.CONV_FP16)
+        ids.append((i, "registered", srv.submit_conv_filter_fp16(f, rows, x16, s_in, fl)))
+        ids.append((None, "fp32", srv.submit_conv(n, rows, x, h, k, fl)))
+        with pytest.raises(estl_amd.EstlError):
+            srv.submit_conv_filter(f, rows, x, fl)          # FP32 request on an FP16 filter
+    srv.drain(120)
+    for i, how, rid in ids:
+        srv.wait(rid, 60)
+        if i is not None:
+            n, rows, k, fl = cases[i]
+            y16, s_out = srv.output_fp16(rid, n, rows)
+            torch.cuda.synchronize()
+            assert torch.equal(y16, refs[i][0]), (cases[i], how)
+            assert torch.equal(s_out, refs[i][1]), (cases[i], how, s_out.tolist(), refs[i][1].tolist())
+        srv.release(rid)
+    assert srv.stats()["failed"] == 0
+    srv.close()
