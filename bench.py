@@ -206,7 +206,7 @@ def bench_large(torch, estl_amd, stream, reps: int = 10, check: bool = True):
     return out
 
 
-def _drive(srv, reqs, xp, hp, arrivals, filters=None, keep=()):
+def _drive(srv, reqs, xp, hp, arrivals, filters=None, keep=(), fp16=None):
     """Submit request i at time arrivals[i] (s from start) -- with its own filter
     (hp[:k], spectrum computed inside the request) or, when `filters` maps (n, bank
     index) to a registered filter, with that -- and release each output as soon as the
@@ -219,7 +219,9 @@ def _drive(srv, reqs, xp, hp, arrivals, filters=None, keep=()):
         now = time.perf_counter() - t0
         while i < count and arrivals[i] <= now:
             n, rows, k, fl = int(reqs.n[i]), int(reqs.rows[i]), int(reqs.k[i]), int(reqs.flags[i])
-            if filters is not None:
+            if fp16 is not None:        # (binary16 input pool, per-row scales)
+                rid = srv.submit_conv_fp16(n, rows, fp16[0], fp16[1], hp, k, fl)
+            elif filters is not None:
                 rid = srv.submit_conv_filter(filters[(n, int(reqs.bank[i]))], rows, xp, fl)
             else:
                 rid = srv.submit_conv(n, rows, xp, hp, k, fl)
@@ -242,8 +244,8 @@ def bench_serve(torch, estl_amd, stream, load: float = 0.7, budget: int = 16 << 
     Poisson arrivals at `load` x capacity: p50/p99 latency (submit -> completion
     observed), throughput, a bitwise alone-vs-mix check; beside two plain forms on one
     stream -- one blocking set_filter + execute per request, and one execute per
-    transform (each real row on its own) -- and the filter-bank form (32 filters
-    registered once per n, resident in the arena)."""
+    transform (each real row on its own) -- the filter-bank form (32 filters
+    registered once per n, resident in the arena) and the FP16-storage mode."""
     import estl_amd.workloads as wl
     reqs = wl.conv_requests()
     count = len(reqs.n)
@@ -296,7 +298,41 @@ def bench_serve(torch, estl_amd, stream, load: float = 0.7, budget: int = 16 << 
     srv.reset_stats()
     wall_bank, _ = _drive(srv, reqs, xp, hp, [0.0] * count, filters)
     bank = srv.stats()
-    failed = cap["failed"] + st["failed"] + bank["failed"]
+    # FP16 storage: the same stream on binary16 inputs (unit row scales: the pool is
+    # already in [-1, 1)), outputs binary16 with per-row scales
+    x16 = xp.to(torch.float16)
+    ones = torch.ones(256, dtype=torch.float32, device="cuda:0")
+    for j in shapes.values():
+        rid = srv.submit_conv_fp16(int(reqs.n[j]), int(reqs.rows[j]), x16, ones, hp, int(reqs.k[j]),
+                                   int(reqs.flags[j]))
+        srv.wait(rid, 60)
+        srv.release(rid)
+    srv.next_done(4096, 0.0)
+    srv.reset_stats()
+    wall_cap16, _ = _drive(srv, reqs, xp, hp, [0.0] * count, fp16=(x16, ones))
+    cap16 = srv.stats()
+    rate16 = load * count / wall_cap16
+    arrivals16 = (reqs.gaps / rate16).cumsum()
+    srv.reset_stats()
+    wall16, held16 = _drive(srv, reqs, xp, hp, arrivals16, keep=sample, fp16=(x16, ones))
+    st16 = srv.stats()
+    same16 = True
+    for j, rid in held16.items():
+        n, rows, k, fl = int(reqs.n[j]), int(reqs.rows[j]), int(reqs.k[j]), int(reqs.flags[j])
+        y16, s16 = srv.output_fp16(rid, n, rows)
+        c = estl_amd.Convolution(n, rows, fl, precision=estl_amd.FP16)
+        c.set_filter(hp, k)
+        z16, t16 = torch.empty_like(y16), torch.empty_like(s16)
+        c.execute_fp16(x16, ones, z16, t16)
+        torch.cuda.synchronize()
+        same16 = same16 and torch.equal(y16, z16) and torch.equal(s16, t16)
+        srv.release(rid)
+    alg16 = float(4 * reqs.values().sum() + 4 * reqs.k.sum())
+    fp16_mode = {"capacity_req_s": round(count / wall_cap16, 1), "capacity_gbs": round(alg16 / wall_cap16 / 1e9, 1),
+                 "offered_req_s": round(rate16, 1), "p50_ms": st16["p50_ms"], "p99_ms": st16["p99_ms"],
+                 "throughput_req_s": round(count / wall16, 1), "alone_equals_mix": same16,
+                 "bytes_per_value": 4, "scaling": "per-row input scales; output scale = input scale x sum|h|"}
+    failed = cap["failed"] + st["failed"] + bank["failed"] + cap16["failed"] + st16["failed"]
     srv.close()
     # plain form: one blocking call per request (objects per shape built beforehand)
     objs = {key: estl_amd.Convolution(key[0], key[1], key[2]) for key in shapes}
@@ -355,7 +391,7 @@ def bench_serve(torch, estl_amd, stream, load: float = 0.7, budget: int = 16 << 
             "served_over_plain_per_transform": round(wall_pt / wall_cap, 2),
             "filter_bank_capacity_req_s": round(count / wall_bank, 1), "filter_bank_registered": len(filters),
             "arena_peak_bytes": st["arena_peak_bytes"], "peak_inflight": st["peak_inflight"],
-            "rejected": st["rejected"], "failed": failed, "alone_equals_mix": same}
+            "rejected": st["rejected"], "failed": failed, "alone_equals_mix": same, "fp16_storage": fp16_mode}
 
 
 def main() -> int:
