@@ -194,13 +194,14 @@ def bench_large(torch, estl_amd, stream, reps: int = 10, check: bool = True):
     s_in = torch.empty(1, dtype=torch.float32, device="cuda:0")
     s_out = torch.empty_like(s_in)
     estl_amd.pack_fp16(x, x16, s_in, n, 1, stream)
-    p16 = estl_amd.Plan(n, 1, estl_amd.FORWARD, estl_amd.FP16)
+    p16 = estl_amd.Plan(n, 1, estl_amd.FORWARD, estl_amd.FP16, estl_amd.FLAG_AUTOTUNE)
     ms = timed(lambda: p16.execute_fp16(x16, s_in, y16, s_out, stream))
     estl_amd.unpack_fp16(y16, s_out, y, n, 1, stream)
     stream.synchronize()
     out["fp16_storage"] = {"ms": round(ms, 4), "gbs": round(8.0 * n / (ms * 1e-3) / 1e9, 1),
                            "fp32_equivalent_gbs": round(16.0 * n / (ms * 1e-3) / 1e9, 1),
                            "scaling": "per-transform max-abs input scale, 1/sqrt(radix) per pass",
+                           "passes": [[q["radix"], q["tile"]] for q in p16.describe().get("passes", [])],
                            "rel_l2_vs_fp64": oracle.rel_l2(y.cpu().numpy(), ref) if check else None}
     out["bound_fp32"] = 1e-6 * 26
     return out

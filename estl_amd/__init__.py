@@ -81,6 +81,7 @@ def lib():
     _sig(L, "estl_plan_kernel_ms", ctypes.c_double, [vp, ci])
     _sig(L, "estl_plan_kernel_info", ci, [vp, ci, ctypes.c_char_p, ci])
     _sig(L, "estl_tune", ci, [cl, cl, ci, ctypes.c_char_p, ci])
+    _sig(L, "estl_tune_precision", ci, [cl, cl, ci, ci, ctypes.c_char_p, ci])
     lp = ctypes.POINTER(ctypes.c_long)
     _sig(L, "estl_ubatch_create", ci, [ctypes.POINTER(vp), ci, lp, lp, ci, cu])
     _sig(L, "estl_ubatch_execute", ci, [vp, vp, vp, vp])
@@ -475,10 +476,10 @@ def fill_uniform(t, seed: int, offset: int = 0, stream=None) -> None:
     check(lib().estl_fill_uniform(t.data_ptr(), count, seed, offset, _stream_ptr(stream)))
 
 
-def tune(n: int, batch: int = 1, direction: int = FORWARD) -> dict:
-    """Autotune the plan for (n, batch, direction); returns every candidate's time."""
+def tune(n: int, batch: int = 1, direction: int = FORWARD, precision: int = FP32) -> dict:
+    """Autotune the plan for (n, batch, direction, precision); returns every candidate's time."""
     buf = ctypes.create_string_buffer(1 << 16)
-    check(lib().estl_tune(n, batch, direction, buf, 1 << 16))
+    check(lib().estl_tune_precision(n, batch, direction, precision, buf, 1 << 16))
     return json.loads(buf.value.decode())
 
 

@@ -218,7 +218,7 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
         st.args.xcd_remap = 0;
         if (tiles && tiles->size() == bits.size()) {
             const int lt = (*tiles)[q];
-            if (lt < 0 || lt > kMaxPassCols || lt + bits[q] > (fp16 ? 13 : kMaxPassTileLog2) || lt > p->log2n - bits[q])
+            if (lt < 0 || lt > kMaxPassCols || lt + bits[q] > kMaxPassTileLog2 || lt > p->log2n - bits[q])
                 return set_error(ESTL_ERR_UNSUPPORTED, "tile 2^%d invalid for pass radix 2^%d", lt, bits[q]);
             st.log2t = lt;
             // tiles narrower than a 128-byte line share lines with their neighbours
@@ -380,11 +380,12 @@ int estl_plan_create(estl_plan_t *plan, long n, long batch, int direction, int p
     *plan = nullptr;
     Plan *p = new Plan();
     int rc;
-    if ((flags & ESTL_FLAG_AUTOTUNE) && n >= 2 && !(n & (n - 1)) && precision == ESTL_PRECISION_FP32 &&
+    if ((flags & ESTL_FLAG_AUTOTUNE) && n >= 2 && !(n & (n - 1)) &&
+        (precision == ESTL_PRECISION_FP32 || precision == ESTL_PRECISION_FP16) &&
         (direction == ESTL_FORWARD || direction == ESTL_BACKWARD) && batch >= 1) {
         bool rows = false;
         std::vector<int> split, tiles;
-        rc = tune_plan(n, batch, direction, &rows, &split, &tiles, nullptr);
+        rc = tune_plan(n, batch, direction, precision, &rows, &split, &tiles, nullptr);
         if (!rc) rc = build_plan(p, n, batch, direction, precision, flags, rows ? nullptr : &split, rows ? nullptr : &tiles);
     } else {
         rc = build_plan(p, n, batch, direction, precision, flags, nullptr);

@@ -24,26 +24,51 @@ __global__ void k_zero_u32(unsigned *__restrict__ p, long count) {
     if (i < count) p[i] = 0u;
 }
 
-// Thread gid handles EPT = min(16, n) consecutive values of one transform; the n / EPT
-// threads of a transform are consecutive and (a power of two) aligned, so xor
-// shuffles inside groups of min(n / EPT, 64) lanes never mix transforms.  Every thread
-// of the launch takes part in the shuffles (threads past the end contribute 0).
-__global__ void k_maxabs(const float2 *__restrict__ x, int log2n, long batch, unsigned *__restrict__ umax) {
-    const int ept_log2 = log2n < 4 ? log2n : 4;
-    const int lpt_log2 = log2n - ept_log2;  // lanes per transform
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long threads = batch << lpt_log2;
+// Max-abs per transform.  A workgroup of 256 threads takes a chunk of 2^c consecutive
+// values, c = min(log2 n, 14): a piece of one transform, or whole transforms when
+// n < 2^14.  Each thread reduces 2^(c-8) values (1 when c < 8); the threads of
+// one transform then combine through xor shuffles inside the wave and, for chunks that
+// span waves, through LDS; one atomicMax per transform and chunk (max is order-free,
+// so the result does not depend on timing).
+constexpr int kMaxAbsChunkLog2 = 14;
+
+__global__ void __launch_bounds__(256) k_maxabs(const float2 *__restrict__ x, int log2n, long batch,
+                                                unsigned *__restrict__ umax) {
+    __shared__ float wmax[4];
+    const int clog = log2n < kMaxAbsChunkLog2 ? log2n : kMaxAbsChunkLog2;  // chunk = min(n, 2^14)
+    const int per_log = clog > 8 ? clog - 8 : 0;                            // values per thread
+    const int tpt_log = log2n - per_log < 8 ? log2n - per_log : 8;          // threads per transform in this block
+    const long base = (long)blockIdx.x << (per_log + 8);                    // first value of the block
+    const long total = batch << log2n;
+    const int t = threadIdx.x;
     float m = 0.0f;
-    if (gid < threads) {
-        const float2 *p = x + (gid << ept_log2);
-        for (int e = 0; e < (1 << ept_log2); ++e) {
-            const float2 v = p[e];
+    // thread t reads values base + t + 256 e (coalesced); a block of 2^(per_log + 8) > 256
+    // values lies inside one transform, so all of them belong to first's transform
+    const long first = base + t;
+    if (first < total) {
+        const float2 *p = x + first;
+        for (int e = 0; e < (1 << per_log); ++e) {
+            const float2 v = p[(long)e << 8];
             m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
         }
     }
-    const int seg = lpt_log2 < 6 ? (1 << lpt_log2) : 64;
+    // lanes of one transform: groups of 2^tpt_log consecutive threads
+    const int seg = tpt_log < 6 ? (1 << tpt_log) : 64;
     for (int off = 1; off < seg; off <<= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if (gid < threads && (gid & (seg - 1)) == 0) atomicMax(umax + (gid >> lpt_log2), __float_as_uint(m));
+    if (tpt_log > 6) {  // the transform spans the block's waves
+        if ((t & 63) == 0) wmax[t >> 6] = m;
+        __syncthreads();
+        if (t == 0) {
+            float mm = wmax[0];
+            for (int w = 1; w < (1 << (tpt_log - 6)); ++w) mm = fmaxf(mm, wmax[w]);
+            if (first < total) atomicMax(umax + (first >> log2n), __float_as_uint(mm));
+        }
+        if (tpt_log == 7) {  // two transforms per block: the second half's leader
+            if (t == 128 && first < total) atomicMax(umax + (first >> log2n), __float_as_uint(fmaxf(wmax[2], wmax[3])));
+        }
+    } else if (first < total && (t & (seg - 1)) == 0) {
+        atomicMax(umax + (first >> log2n), __float_as_uint(m));
+    }
 }
 
 // y = x / m_t in FP16; scale[t] = m_t (1 for an all-zero transform).  `scale` holds the
@@ -89,9 +114,10 @@ hipError_t launch_pack_fp16(const float2 *x, void *y16, float *scale, int log2n,
     f16x2 *y = static_cast<f16x2 *>(y16);
     unsigned *umax = reinterpret_cast<unsigned *>(scale);
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(batch, 256)), dim3(256), 0, st, umax, batch);
-    const int ept_log2 = log2n < 4 ? log2n : 4;
-    const long threads = batch << (log2n - ept_log2);
-    hipLaunchKernelGGL(k_maxabs, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, x, log2n, batch, umax);
+    const int clog = log2n < kMaxAbsChunkLog2 ? log2n : kMaxAbsChunkLog2;
+    const int per_log = clog > 8 ? clog - 8 : 0;
+    const long blocks = ((batch << log2n) + (1L << (per_log + 8)) - 1) >> (per_log + 8);
+    hipLaunchKernelGGL(k_maxabs, dim3((unsigned)blocks), dim3(256), 0, st, x, log2n, batch, umax);
     hipLaunchKernelGGL(k_pack, dim3(grid_for(batch << log2n, 256)), dim3(256), 0, st, x, y, log2n, batch, scale);
     return hipGetLastError();
 }
@@ -197,12 +223,12 @@ k_fft_pass_h(const f16x2 *__restrict__ in, f16x2 *__restrict__ out, PassArgs a) 
 }
 
 // FP16 passes: radices 2^4 .. 2^9 (every split the default planner makes), tiles of at
-// most 2^13 values
+// most 2^14 values
 constexpr int kMaxPassLog2Fp16 = 9;
 
 template <int LOG2B, int DIR, int LOG2T, bool MID>
 static hipError_t launch_pass_h_t(const f16x2 *in, f16x2 *out, const PassArgs &a, hipStream_t st) {
-    if constexpr (LOG2B + LOG2T > 13) {
+    if constexpr (LOG2B + LOG2T > kMaxPassTileLog2) {
         return hipErrorInvalidValue;
     } else {
         const int lcpr = a.log2n - LOG2B;

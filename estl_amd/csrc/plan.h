@@ -60,7 +60,7 @@ int ensure_scratch(Plan *p, int which);
 void destroy_plan(Plan *p);
 std::string describe_plan(const Plan *p);
 int plan_launches(const Plan *p);
-int tune_plan(long n, long batch, int dir, bool *rows, std::vector<int> *split, std::vector<int> *tiles,
+int tune_plan(long n, long batch, int dir, int precision, bool *rows, std::vector<int> *split, std::vector<int> *tiles,
               std::string *report);
 
 // ---- batched circular convolution: immutable shape + per-execution buffers

@@ -118,7 +118,7 @@ void compositions(int L, int P, int lo, int hi, std::vector<int> &cur, std::vect
     }
 }
 
-std::vector<Choice> candidates(int log2n) {
+std::vector<Choice> candidates(int log2n, int max_b) {
     std::vector<Choice> out;
     if (log2n <= kMaxRowsLog2) {
         Choice c;
@@ -127,8 +127,8 @@ std::vector<Choice> candidates(int log2n) {
     }
     const int p0 = std::max(2, (log2n + 8) / 9);
     for (int P = std::max(2, p0 - 1); P <= std::min(4, p0 + 1); ++P) {
-        if (P * 4 > log2n || P * kMaxPassLog2 < log2n) continue;
-        const int lo = std::max(4, log2n / P - 1), hi = std::min(kMaxPassLog2, (log2n + P - 1) / P + 1);
+        if (P * 4 > log2n || P * max_b < log2n) continue;
+        const int lo = std::max(4, log2n / P - 1), hi = std::min(max_b, (log2n + P - 1) / P + 1);
         std::vector<std::vector<int>> comps;
         std::vector<int> cur;
         compositions(log2n, P, lo, hi, cur, comps);
@@ -145,18 +145,22 @@ std::vector<Choice> candidates(int log2n) {
 
 int default_pass_tile(int log2b, int log2cpr, long total_cols);
 
-// Time one candidate: median-free minimum of `reps` runs after a warm-up.
-static int time_candidate(const Choice &c, long n, long batch, int dir, float2 *in, float2 *out, hipStream_t st,
-                          hipEvent_t e0, hipEvent_t e1, double *us) {
+int execute_plan_fp16(Plan *p, const void *in, const float *in_scale, void *out, float *out_scale, hipStream_t st);
+
+// Time one candidate: median-free minimum of `reps` runs after a warm-up.  FP16
+// candidates run through the FP16 entry point (`scales`: two arrays of batch floats).
+static int time_candidate(const Choice &c, long n, long batch, int dir, int precision, float2 *in, float2 *out,
+                          float *scales, hipStream_t st, hipEvent_t e0, hipEvent_t e1, double *us) {
     Plan p;
-    int rc = build_plan(&p, n, batch, dir, ESTL_PRECISION_FP32, 0, c.rows ? nullptr : &c.split,
+    int rc = build_plan(&p, n, batch, dir, precision, 0, c.rows ? nullptr : &c.split,
                         c.tiles.empty() ? nullptr : &c.tiles);
     if (rc) return rc;
     double best = 1e30;
     const int reps = 4;
     for (int r = 0; r < reps + 1; ++r) {
         if (hipEventRecord(e0, st) != hipSuccess) return set_error(ESTL_ERR_HIP, "event record failed");
-        rc = execute_plan(&p, in, out, st);
+        rc = precision == ESTL_PRECISION_FP16 ? execute_plan_fp16(&p, in, scales, out, scales + batch, st)
+                                              : execute_plan(&p, in, out, st);
         if (rc) break;
         if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
             return set_error(ESTL_ERR_HIP, "event sync failed");
@@ -174,13 +178,17 @@ static int time_candidate(const Choice &c, long n, long batch, int dir, float2 *
 // Tune (or look up) the plan for (n, batch, dir) on the current device.  On success
 // fills split/tiles (empty split = rows kernel) and, if report is given, a JSON report
 // of every candidate's time.
-int tune_plan(long n, long batch, int dir, bool *rows, std::vector<int> *split, std::vector<int> *tiles,
-              std::string *report) {
+int tune_plan(long n, long batch, int dir, int precision, bool *rows, std::vector<int> *split,
+              std::vector<int> *tiles, std::string *report) {
     if (n < 2 || (n & (n - 1))) return set_error(ESTL_ERR_INVALID, "n=%ld is not a power of two >= 2", n);
     int log2n = 0;
     while ((1L << log2n) < n) ++log2n;
     const int device = current_device();
-    const Key key(device_arch(device), log2n, batch, dir);
+    if (precision != ESTL_PRECISION_FP32 && precision != ESTL_PRECISION_FP16)
+        return set_error(ESTL_ERR_UNSUPPORTED, "precision %d not supported", precision);
+    const bool fp16 = precision == ESTL_PRECISION_FP16;
+    // FP16 plans are cached under the architecture name plus "+fp16"
+    const Key key(device_arch(device) + (fp16 ? "+fp16" : ""), log2n, batch, dir);
     {
         std::lock_guard<std::mutex> lk(g_mu);
         load_cache_locked();
@@ -194,26 +202,30 @@ int tune_plan(long n, long batch, int dir, bool *rows, std::vector<int> *split, 
     }
     const size_t bytes = sizeof(float2) * (size_t)n * (size_t)batch;
     float2 *in = nullptr, *out = nullptr;
+    float *scales = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = ESTL_OK;
     auto cleanup = [&] {
         if (in) (void)hipFree(in);
         if (out) (void)hipFree(out);
+        if (scales) (void)hipFree(scales);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (st) (void)hipStreamDestroy(st);
     };
-    if (hipMalloc(&in, bytes) != hipSuccess || hipMalloc(&out, bytes) != hipSuccess) {
+    if (hipMalloc(&in, bytes) != hipSuccess || hipMalloc(&out, bytes) != hipSuccess ||
+        hipMalloc(&scales, 2 * sizeof(float) * (size_t)batch) != hipSuccess) {
         cleanup();
         return set_error(ESTL_ERR_NOMEM, "tuner could not allocate 2 x %zu bytes", bytes);
     }
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
-        hipEventCreate(&e1) != hipSuccess || hipMemsetAsync(in, 0, bytes, st) != hipSuccess) {
+        hipEventCreate(&e1) != hipSuccess || hipMemsetAsync(in, 0, bytes, st) != hipSuccess ||
+        hipMemsetAsync(scales, 0, 2 * sizeof(float) * (size_t)batch, st) != hipSuccess) {
         cleanup();
         return set_error(ESTL_ERR_HIP, "tuner set-up failed");
     }
-    std::vector<Choice> cands = candidates(log2n);
+    std::vector<Choice> cands = candidates(log2n, fp16 ? max_pass_log2_fp16() : kMaxPassLog2);
     Choice best;
     best.us = 1e30;
     std::string rep = "{\"n\": " + std::to_string(n) + ", \"batch\": " + std::to_string(batch) +
@@ -221,7 +233,7 @@ int tune_plan(long n, long batch, int dir, bool *rows, std::vector<int> *split, 
     bool first = true;
     for (auto &c : cands) {
         double us = 0;
-        rc = time_candidate(c, n, batch, dir, in, out, st, e0, e1, &us);
+        rc = time_candidate(c, n, batch, dir, precision, in, out, scales, st, e0, e1, &us);
         if (rc) continue;  // a split without kernels for this size: skip it
         c.us = us;
         if (!c.rows) {
@@ -250,7 +262,7 @@ int tune_plan(long n, long batch, int dir, bool *rows, std::vector<int> *split, 
                 Choice c = best;
                 c.tiles[q] = lt;
                 double us = 0;
-                if (time_candidate(c, n, batch, dir, in, out, st, e0, e1, &us)) continue;
+                if (time_candidate(c, n, batch, dir, precision, in, out, scales, st, e0, e1, &us)) continue;
                 char buf[256];
                 std::string s = "[";
                 for (size_t k = 0; k < c.split.size(); ++k) s += (k ? ", " : "") + std::to_string(c.split[k]);
@@ -286,10 +298,14 @@ int tune_plan(long n, long batch, int dir, bool *rows, std::vector<int> *split, 
 using namespace estl;
 
 extern "C" int estl_tune(long n, long batch, int direction, char *report, int len) {
+    return estl_tune_precision(n, batch, direction, ESTL_PRECISION_FP32, report, len);
+}
+
+extern "C" int estl_tune_precision(long n, long batch, int direction, int precision, char *report, int len) {
     bool rows = false;
     std::vector<int> split, tiles;
     std::string rep;
-    const int rc = tune_plan(n, batch, direction, &rows, &split, &tiles, &rep);
+    const int rc = tune_plan(n, batch, direction, precision, &rows, &split, &tiles, &rep);
     if (rc) return rc;
     if (report && len > 0) {
         strncpy(report, rep.c_str(), (size_t)len - 1);

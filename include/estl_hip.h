@@ -95,6 +95,9 @@ int estl_plan_create_split(estl_plan_t *plan, long n, long batch, int direction,
  * FFT_OpenCL_Contiguous.h:55-60. */
 int estl_tune(long n, long batch, int direction, char *report, int len);
 
+/* The same for a given precision (ESTL_PRECISION_FP16 tunes the FP16-storage plan). */
+int estl_tune_precision(long n, long batch, int direction, int precision, char *report, int len);
+
 /* Destroy a plan (FFT_OpenCL_Contiguous::~FFT_OpenCL_Contiguous, FFT_OpenCL_Contiguous.h:186-191). */
 int estl_plan_destroy(estl_plan_t plan);
 
