@@ -24,6 +24,7 @@ FP32 = 0
 FP16 = 1
 FLAG_NORMALIZE = 0x1
 FLAG_AUTOTUNE = 0x2
+FLAG_PLANAR = 0x4
 CONV_CORRELATE = 0x10
 CONV_NORMALIZE_KERNEL = 0x20
 CONV_CENTER = 0x40

@@ -170,6 +170,8 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
     if (precision != ESTL_PRECISION_FP32 && precision != ESTL_PRECISION_FP16)
         return set_error(ESTL_ERR_UNSUPPORTED, "precision %d not supported", precision);
     const bool fp16 = precision == ESTL_PRECISION_FP16;
+    const bool planar = (flags & ESTL_FLAG_PLANAR) != 0;
+    if (planar && fp16) return set_error(ESTL_ERR_UNSUPPORTED, "the tile-planarized layout is FP32 only");
     p->device = current_device();
     p->n = n;
     p->batch = batch;
@@ -238,6 +240,10 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
         p->steps.push_back(st);
         A *= B;
     }
+    if (planar) {  // the first pass reads, the last writes the tile-planarized layout
+        p->steps.front().args.planar_in = 1;
+        p->steps.back().args.planar_out = 1;
+    }
     p->scratch_bytes = (fp16 ? 4 : sizeof(float2)) * (size_t)n * (size_t)batch;
     return ESTL_OK;
 }
@@ -260,12 +266,15 @@ int execute_plan_ws(Plan *p, const float2 *in, float2 *out, float2 *ws0, float2 
         return set_error(ESTL_ERR_INVALID, "an FP16-storage plan runs through estl_execute_fp16");
     switch (p->kind) {
         case Plan::COPY: {
-            if (p->scale == 1.0f && in != out)
-                HIP_TRY(hipMemcpyAsync(out, in, sizeof(float2) * p->batch, hipMemcpyDeviceToDevice, st));
+            const size_t bytes = (p->flags & ESTL_FLAG_PLANAR) ? planar_bytes(p->batch) : sizeof(float2) * p->batch;
+            if (p->scale == 1.0f && in != out) HIP_TRY(hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, st));
             return ESTL_OK;
         }
         case Plan::ROWS:
-            HIP_TRY(launch_rows(p->log2n, p->dir, in, out, p->batch, p->tw_rows, p->scale, st));
+            if (p->flags & ESTL_FLAG_PLANAR)
+                HIP_TRY(launch_rows_planar(p->log2n, p->dir, in, out, p->batch, p->tw_rows, p->scale, st));
+            else
+                HIP_TRY(launch_rows(p->log2n, p->dir, in, out, p->batch, p->tw_rows, p->scale, st));
             return ESTL_OK;
         case Plan::PASSES: {
             const int P = (int)p->steps.size();
@@ -431,7 +440,8 @@ int estl_execute_device(estl_plan_t plan, const void *d_in, void *d_out, void *s
 int estl_execute_host(estl_plan_t plan, const void *h_in, void *h_out) {
     if (!plan || !h_in || !h_out) return set_error(ESTL_ERR_INVALID, "null argument");
     Plan *p = reinterpret_cast<Plan *>(plan);
-    const size_t bytes = sizeof(float2) * (size_t)p->n * (size_t)p->batch;
+    const long count = p->n * p->batch;
+    const size_t bytes = (p->flags & ESTL_FLAG_PLANAR) ? planar_bytes(count) : sizeof(float2) * (size_t)count;
     if (!p->stream) HIP_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     if (!p->h_in_dev) HIP_TRY(hipMalloc(&p->h_in_dev, bytes));
     if (!p->h_out_dev) HIP_TRY(hipMalloc(&p->h_out_dev, bytes));

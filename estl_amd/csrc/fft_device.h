@@ -53,6 +53,20 @@ __device__ __forceinline__ void st_stream(float2 *p, float2 x) {
 __device__ __forceinline__ float ld_stream_f(const float *p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st_stream_f(float *p, float x) { __builtin_nontemporal_store(x, p); }
 
+// The reference's tile-planarized layout (PlannarizedComplexArray<5>, Complex.h:34-48,
+// 97-130): in a float array, the real part of value e sits at ((e >> 5) << 6) | (e & 31)
+// and its imaginary part 32 floats later -- 32 reals, then their 32 imaginaries.
+__device__ __forceinline__ long planar_pos(long e) { return ((e >> 5) << 6) | (e & 31); }
+__device__ __forceinline__ float2 ld_planar(const float2 *base, long e) {
+    const float *f = reinterpret_cast<const float *>(base) + planar_pos(e);
+    return make_float2(__builtin_nontemporal_load(f), __builtin_nontemporal_load(f + 32));
+}
+__device__ __forceinline__ void st_planar(float2 *base, long e, float2 v) {
+    float *f = reinterpret_cast<float *>(base) + planar_pos(e);
+    __builtin_nontemporal_store(v.x, f);
+    __builtin_nontemporal_store(v.y, f + 32);
+}
+
 // FP16 storage of a complex value (re, im as IEEE binary16), computed on in FP32.
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float2 ld_elem(const float2 *p) { return ld_stream(p); }

@@ -29,11 +29,16 @@ struct PassArgs {
     float *rout;
     const void *rin16;       // the same, binary16 reals (FP16-storage passes)
     void *rout16;
+    int planar_in;           // `in` / `out` use the tile-planarized layout (FP32 only)
+    int planar_out;
     const float2 *spectrum;
 };
 
 hipError_t launch_rows(int log2n, int dir, const float2 *in, float2 *out, long batch, const float2 *tw, float scale,
                        hipStream_t st);
+// rows kernel on the tile-planarized layout (kernels.hip)
+hipError_t launch_rows_planar(int log2n, int dir, const float2 *in, float2 *out, long batch, const float2 *tw,
+                              float scale, hipStream_t st);
 hipError_t launch_pass_fwd(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st);
 hipError_t launch_pass_bwd(int log2b, int log2t, const float2 *in, float2 *out, const PassArgs &a, hipStream_t st);
 // FP16 storage (fp16.hip): buffers of (re, im) binary16 pairs, passed untyped

@@ -9,6 +9,10 @@
 //                  (replaces the per-element native_cos/native_sin of
 //                  KernelGenerator.h:48)
 //   k_fill_uniform seeded synthetic input (splitmix64 stream)
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "fft_block.h"
 #include "internal.h"
 
@@ -119,22 +123,57 @@ k_fft_rows(const float2 *__restrict__ in, float2 *__restrict__ out, long batch,
     }
 }
 
+// Rows on the reference's tile-planarized layout (PlannarizedComplexArray<5>): the same
+// transform, loads and stores by element index through planar_pos.  Persistent over
+// groups of TPW rows; no prefetch (this layout is an interface adapter, not the
+// throughput path).
+template <int LOG2N, int DIR, int TPW>
+__global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT * TPW)
+k_fft_rows_planar(const float2 *__restrict__ in, float2 *__restrict__ out, long batch, const float2 *__restrict__ tw,
+                  float scale) {
+    using G = BlockGeom<LOG2N>;
+    constexpr int N = G::N, E = G::E, TPT = G::TPT;
+    constexpr int SLICE = G::uses_lds ? Lds<kLdsPad>::size(N) : 0;
+    constexpr int LDS = G::uses_lds ? TPW * SLICE : 1;
+    __shared__ float2 lds[LDS];
+    const long groups = (batch + TPW - 1) / TPW;
+    bool busy = false;
+    for (long grp = blockIdx.x; grp < groups; grp += gridDim.x) {  // uniform across the workgroup
+        const int t = opaque_lane(threadIdx.x);
+        const int w = t / TPT, tl = t % TPT;
+        const long want = grp * TPW + w;
+        const long row = want < batch ? want : batch - 1;
+        float2 v[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) v[i] = ld_planar(in, row * N + tl + TPT * i);
+        block_fft<LOG2N, DIR, kLdsPad>(v, lds + w * SLICE, tl, tw, busy);
+        busy = true;
+        if (want < batch) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) st_planar(out, row * N + tl + TPT * i, cscale(v[i], scale));
+        }
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 // Workgroups of `kernel` resident on the whole device at once (occupancy x CUs),
-// cached per kernel and device.
+// cached per kernel (by address: kernels of one signature share a pointer type) and
+// device.
 template <class K>
 static long resident_blocks(K kernel, int block_threads) {
-    static long cache[64] = {0};
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, long> cache;
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if (cache[dev] == 0) {
-        int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block_threads, 0) != hipSuccess || per_cu < 1)
-            per_cu = 1;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
-        cache[dev] = (long)per_cu * cus;
-    }
-    return cache[dev];
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    const auto key = std::make_pair(reinterpret_cast<const void *>(kernel), dev);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block_threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    return cache[key] = (long)per_cu * cus;
 }
 
 // persistent grid: this many resident rounds of workgroups
@@ -170,6 +209,38 @@ hipError_t launch_rows(int log2n, int dir, const float2 *in, float2 *out, long b
                        hipStream_t st) {
     return dir < 0 ? launch_rows_d<-1>(log2n, in, out, batch, tw, scale, st)
                    : launch_rows_d<+1>(log2n, in, out, batch, tw, scale, st);
+}
+
+template <int LOG2N, int DIR>
+static hipError_t launch_rows_planar_t(const float2 *in, float2 *out, long batch, const float2 *tw, float scale,
+                                       hipStream_t st) {
+    using G = BlockGeom<LOG2N>;
+    constexpr int TPW = (G::TPT >= 256) ? 1 : 256 / G::TPT;
+    auto kernel = k_fft_rows_planar<LOG2N, DIR, TPW>;
+    const long groups = (batch + TPW - 1) / TPW;
+    long blocks = resident_blocks(kernel, G::TPT * TPW) * kRowsGridRounds;
+    if (blocks > groups) blocks = groups;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(G::TPT * TPW), 0, st, in, out, batch, tw, scale);
+    return hipGetLastError();
+}
+
+template <int DIR>
+static hipError_t launch_rows_planar_d(int log2n, const float2 *in, float2 *out, long batch, const float2 *tw,
+                                       float scale, hipStream_t st) {
+    switch (log2n) {
+#define ROWS_P_CASE(L) \
+    case L: return launch_rows_planar_t<L, DIR>(in, out, batch, tw, scale, st);
+        ROWS_P_CASE(1) ROWS_P_CASE(2) ROWS_P_CASE(3) ROWS_P_CASE(4) ROWS_P_CASE(5) ROWS_P_CASE(6) ROWS_P_CASE(7)
+        ROWS_P_CASE(8) ROWS_P_CASE(9) ROWS_P_CASE(10) ROWS_P_CASE(11) ROWS_P_CASE(12) ROWS_P_CASE(13)
+#undef ROWS_P_CASE
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_rows_planar(int log2n, int dir, const float2 *in, float2 *out, long batch, const float2 *tw,
+                              float scale, hipStream_t st) {
+    return dir < 0 ? launch_rows_planar_d<-1>(log2n, in, out, batch, tw, scale, st)
+                   : launch_rows_planar_d<+1>(log2n, in, out, batch, tw, scale, st);
 }
 
 long rows_twiddle_count(int log2n) {

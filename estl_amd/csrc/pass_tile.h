@@ -89,6 +89,14 @@ __device__ __forceinline__ void pass_tile_load(const ST *__restrict__ in, const 
                 v[i] = make_float2((float)x1[o], two ? (float)x1[(1L << a.log2n) + o] : 0.0f);
             }
         }
+    } else if (F32 && a.planar_in) {
+        // the same values from the tile-planarized layout, by element index
+        const long e0 = (P.row << a.log2n) +
+                        (ls >= LOG2T ? long((P.g << (ls + LOG2B)) + P.z) : long(P.j0 << LOG2B) + P.t);
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            v[i] = ld_planar(reinterpret_cast<const float2 *>(in),
+                             e0 + (ls >= LOG2T ? long((P.tl + TPT * i) << ls) : long(NT * i)));
     } else if (ls >= LOG2T) {
         const ST *p = in + (P.row << a.log2n) + ((P.g << (ls + LOG2B)) + P.z);
 #pragma unroll
@@ -200,6 +208,13 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
             v[i] = a.corr ? cmulc(v[i], hk) : cmul(v[i], hk);
         }
         block_fft<LOG2B, -DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, true);
+    }
+    if (F32 && a.planar_out) {
+        const long e0 = (P.row << a.log2n) + j;
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            st_planar(reinterpret_cast<float2 *>(out), e0 + ((long)(tl + TPT * i) << lcpr), cscale(v[i], a.scale));
+        return;
     }
     ST *dst = out + (P.row << a.log2n) + j;
 #pragma unroll

@@ -39,6 +39,16 @@ struct Plan {
 
 int set_error(int code, const char *fmt, ...);
 
+// bytes that `count` complex values span in the tile-planarized layout: up to the
+// last value's imaginary part (whole tiles when count is a multiple of 32; a partial
+// last tile ends early, so a host array sized by the reference's
+// PlannarizedComplexArray rounding is never overrun)
+inline size_t planar_bytes(long count) {
+    if (count < 1) return 0;
+    const long last = count - 1;
+    return (size_t)((((last >> 5) << 6) | (last & 31)) + 33) * sizeof(float);
+}
+
 // return ESTL_ERR_HIP with the failing call's text from the enclosing function
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \

@@ -66,10 +66,12 @@ std::string plan_kernel_info(const Plan *p, int q) {
 // One launch q of the plan from src to dst (the same launches execute_plan makes).
 static hipError_t launch_one(Plan *p, int q, const void *src, void *dst, hipStream_t st) {
     const bool h = p->precision == ESTL_PRECISION_FP16;
-    if (p->kind == Plan::ROWS)
-        return h ? launch_rows_fp16(p->log2n, p->dir, src, dst, p->batch, p->tw_rows, p->scale, st)
-                 : launch_rows(p->log2n, p->dir, static_cast<const float2 *>(src), static_cast<float2 *>(dst), p->batch,
-                               p->tw_rows, p->scale, st);
+    if (p->kind == Plan::ROWS) {
+        if (h) return launch_rows_fp16(p->log2n, p->dir, src, dst, p->batch, p->tw_rows, p->scale, st);
+        auto f = (p->flags & ESTL_FLAG_PLANAR) ? launch_rows_planar : launch_rows;
+        return f(p->log2n, p->dir, static_cast<const float2 *>(src), static_cast<float2 *>(dst), p->batch, p->tw_rows,
+                 p->scale, st);
+    }
     const PassStep &s = p->steps[q];
     return h ? launch_pass_fp16(s.log2b, s.log2t, p->dir, src, dst, s.args, st)
              : launch_pass(s.log2b, s.log2t, p->dir, static_cast<const float2 *>(src), static_cast<float2 *>(dst),
@@ -80,11 +82,14 @@ int profile_plan(Plan *p, const void *in, void *out, int reps, std::string *json
     const int K = plan_kernel_count(p);
     if (reps < 1) reps = 1;
     const size_t elem = p->precision == ESTL_PRECISION_FP16 ? 4 : sizeof(float2);
-    const size_t bytes = elem * (size_t)p->n * (size_t)p->batch;
+    const size_t bytes = (p->flags & ESTL_FLAG_PLANAR) ? planar_bytes(p->n * p->batch) : elem * (size_t)p->n * (size_t)p->batch;
     if (!p->stream) HIP_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     if (!in || !out) {
-        if (!p->h_in_dev) HIP_TRY(hipMalloc(&p->h_in_dev, sizeof(float2) * (size_t)p->n * (size_t)p->batch));
-        if (!p->h_out_dev) HIP_TRY(hipMalloc(&p->h_out_dev, sizeof(float2) * (size_t)p->n * (size_t)p->batch));
+        // the same sizes estl_execute_host gives these buffers
+        const size_t hb = (p->flags & ESTL_FLAG_PLANAR) ? planar_bytes(p->n * p->batch)
+                                                        : sizeof(float2) * (size_t)p->n * (size_t)p->batch;
+        if (!p->h_in_dev) HIP_TRY(hipMalloc(&p->h_in_dev, hb));
+        if (!p->h_out_dev) HIP_TRY(hipMalloc(&p->h_out_dev, hb));
         HIP_TRY(hipMemsetAsync(p->h_in_dev, 0, bytes, p->stream));
         in = p->h_in_dev;
         out = p->h_out_dev;

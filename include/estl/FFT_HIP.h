@@ -90,4 +90,41 @@ class FFT_HIP<float> : public FFT<float> {
     estl_plan_t plan;
 };
 
+// The same backend behind PlannarizedDataInterface (DataInterface.h:26-35): the host
+// arrays are PlannarizedComplexArray<GlobalPlannarLevel = 5> tiles (32 reals, then their
+// 32 imaginaries), and the kernels read and write that layout directly
+// (ESTL_FLAG_PLANAR): no repacking on the host or the device.
+template <class D>
+class FFT_HIP_Planar;
+
+template <>
+class FFT_HIP_Planar<float> : public FFT<float> {
+  public:
+    typedef PlannarizedDataInterface<float> DataInterfaceType;
+
+    static std::string getName() { return "HIP FFT (gfx950, planarized tiles)"; }
+    static int getMaxBatchSize(int n) { return FFT_HIP<float>::getMaxBatchSize(n); }
+
+    FFT_HIP_Planar(int iSize, bool iForward = true, int iBatchCount = 1, unsigned flags = 0)
+        : FFT<float>(iSize, iBatchCount), plan(nullptr) {
+        this->dataInterface = arrays = new DataInterfaceType(iSize * iBatchCount);
+        const int rc = estl_plan_create(&plan, iSize, iBatchCount, iForward ? ESTL_FORWARD : ESTL_BACKWARD,
+                                        ESTL_PRECISION_FP32, flags | ESTL_FLAG_PLANAR);
+        if (rc != ESTL_OK) throw std::runtime_error(std::string("FFT_HIP_Planar: ") + estl_last_error());
+    }
+
+    virtual ~FFT_HIP_Planar() { estl_plan_destroy(plan); }
+
+    virtual void execute() {
+        const int rc = estl_execute_host(plan, arrays->in->getData(), arrays->out->getData());
+        if (rc != ESTL_OK) printf("FFT_HIP_Planar::execute failed: %s\n", estl_last_error());
+    }
+
+    int getKernelCount() const { return estl_plan_launches(plan); }
+
+  private:
+    DataInterfaceType *arrays;
+    estl_plan_t plan;
+};
+
 #endif  // ESTL_HIP_FFT_HIP_H

@@ -46,6 +46,9 @@ extern "C" {
 
 #define ESTL_FLAG_NORMALIZE 0x1u   /* multiply the result by 1/n                               */
 #define ESTL_FLAG_AUTOTUNE 0x2u    /* pick the plan by timing candidates on the device (cached) */
+#define ESTL_FLAG_PLANAR 0x4u      /* in and out in the tile-planarized layout of              */
+                                   /* PlannarizedComplexArray<5> (Complex.h:34-48,97-130):     */
+                                   /* per 32 values, 32 reals then 32 imaginaries; FP32 only   */
 
 /* convolution options (estl_conv_create) */
 #define ESTL_CONV_CORRELATE 0x10u        /* multiply by conj(H): correlation, Convolution.h:125-131 */

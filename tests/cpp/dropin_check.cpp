@@ -19,7 +19,9 @@
 //      on seeded random input (the reference's input degenerates to a ramp);
 //   3. a backward transform of the forward result gives M times the input on every
 //      batch row (the reference's batched inverse conjugates row 0 only);
-//   4. the per-kernel profiling report in the format of tests/Test.cpp:86-100.
+//   4. the per-kernel profiling report in the format of tests/Test.cpp:86-100;
+//   1b. the known-answer test again through FFT_HIP_Planar, whose host arrays are
+//      PlannarizedDataInterface tiles read and written by the kernels directly.
 // The 2-D test also runs FFT2D_HIP (the device-resident 2-D transform) on the same
 // input against the naive factory.
 #include <FFT.h>
@@ -111,6 +113,37 @@ int main() {
                 report(worst <= 1e-3, what, worst, 1e-3);
             }
         }
+    }
+
+    // 1b. the same known-answer test through the planarized-tile backend (the host arrays
+    //     are PlannarizedDataInterface tiles; the kernels read and write them directly)
+    {
+        FFTFactory<D> *planar = new FFTFactorySpecific<FFT_HIP_Planar<D> >();
+        for (int b : {0, 3}) {
+            const int B = 1 << b;
+            for (int m = 2; m <= 16; m += 2) {
+                const int M = 1 << m;
+                FFT<D> *fft = planar->newFFT(M, true, B);
+                double worst = 0;
+                for (int bIx = 0; bIx < B; ++bIx) {
+                    const int effK = (1 + bIx) % (M / 2 - 1 > 0 ? M / 2 - 1 : 1) + 1;
+                    for (int q = 0; q < M; ++q) fft->setData(q + bIx * M, C((D)cos(2 * M_PI * ((1.0 * effK * q) / M))));
+                }
+                fft->execute();
+                for (int bIx = 0; bIx < B; ++bIx) {
+                    const int effK = (1 + bIx) % (M / 2 - 1 > 0 ? M / 2 - 1 : 1) + 1;
+                    for (int q = 0; q < M; ++q) {
+                        const C want = ((q == effK) || (q == M - effK)) ? C((D)(M / 2.0)) : C((D)0.0);
+                        worst = std::max(worst, norm2(fft->getData(q + bIx * M) - want) / M);
+                    }
+                }
+                delete fft;
+                char what[96];
+                snprintf(what, sizeof what, "known answer, planarized tiles: M=%d batch=%d", M, B);
+                report(worst <= 1e-3, what, worst, 1e-3);
+            }
+        }
+        delete planar;
     }
 
     // 2. 2-D differential test against the naive factory
