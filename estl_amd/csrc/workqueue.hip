@@ -5,7 +5,7 @@
 // serves mixed sizes with one FFT object -- one plan, one program, one launch per pass --
 // per size (tests/Test.cpp:48-53).  Here a descriptor list holds every transform of the
 // batch and one launch computes them all:
-//   * a work item is either a group of same-size rows (n <= 2^12: 256/TPT rows computed
+//   * a work item is either a group of same-size rows (n <= 2^12: kUThreads/TPT rows computed
 //     like k_fft_rows) or one whole large transform (2^13 <= n <= 2^20: both passes of a
 //     two-pass plan, tile by tile, through a scratch region, with a workgroup barrier
 //     between the passes -- no dependency crosses workgroups);
@@ -20,7 +20,16 @@
 
 namespace estl {
 
-constexpr int kUThreads = 256;
+// workgroup size 2^kUThreadsLog2: small items take 2^kUThreadsLog2 / TPT rows, large
+// items tiles of 2^(kUThreadsLog2 + 4 - B) columns.  Measured on MI355X (8192 sizes
+// 2^U{8..18}): 512 threads at four waves per SIMD 2.53 ms, 256 threads 3.42 ms (its
+// 8-column tiles of radix 2^9 read 64-byte runs), 512 threads at three waves 2.97 ms.
+#ifndef ESTL_UBATCH_TLOG
+#define ESTL_UBATCH_TLOG 9
+#endif
+constexpr int kUThreadsLog2 = ESTL_UBATCH_TLOG;
+constexpr int kUThreads = 1 << kUThreadsLog2;
+static_assert(kUThreadsLog2 + 4 <= kUMaxSmallLog2 + 1, "every large size must fill a workgroup's tile");
 
 // pass split of the large sizes: 2^L = 2^(L/2) x 2^(L - L/2).  Measured on MI355X
 // (1 GiB of one size per launch): the even split beats skewed ones -- a radix-16 first
@@ -38,7 +47,7 @@ struct USmall {
 template <int L>
 struct ULarge {
     static constexpr int B1 = ubatch_split(L), B2 = L - B1;
-    static constexpr int T1 = 12 - B1, T2 = 12 - B2;  // 256 threads: T B / 16 = 256
+    static constexpr int T1 = kUThreadsLog2 + 4 - B1, T2 = kUThreadsLog2 + 4 - B2;  // threads = T B / 16
     static constexpr int lds = PassTile<B1, T1>::lds_size > PassTile<B2, T2>::lds_size ? PassTile<B1, T1>::lds_size
                                                                                         : PassTile<B2, T2>::lds_size;
 };
@@ -59,7 +68,7 @@ constexpr int large_lds_max(std::integer_sequence<int, Ls...>) {
 constexpr int kULdsSmall = small_lds_max(std::make_integer_sequence<int, kUMaxSmallLog2>{});
 constexpr int kULdsLarge = large_lds_max(std::make_integer_sequence<int, kUMaxLog2 - kUMaxSmallLog2>{});
 constexpr int kULds = kULdsSmall > kULdsLarge ? kULdsSmall : kULdsLarge;
-static_assert(kULds * 8 <= 64 * 1024, "work-queue LDS must leave room for several workgroups per CU");
+static_assert(kULds * 8 <= 80 * 1024, "work-queue LDS must leave room for two workgroups per CU");
 
 template <int L, int DIR>
 __device__ __forceinline__ void small_item(const UArgs &a, const UItem &it, float2 *lds) {
@@ -112,8 +121,16 @@ __device__ __forceinline__ void large_item(const UArgs &a, const UItem &it, floa
         pass_tile<B2, DIR, T2>(a.scratch + off, a.out + off, p2, tile, lds, true);
 }
 
+// waves per SIMD the register budget is held to (0: the compiler's choice)
+#ifndef ESTL_UBATCH_WAVES
+#define ESTL_UBATCH_WAVES 4
+#endif
 template <int DIR>
-__global__ void __launch_bounds__(kUThreads) k_ubatch(UArgs a) {
+__global__ void __launch_bounds__(kUThreads)
+#if ESTL_UBATCH_WAVES
+__attribute__((amdgpu_waves_per_eu(ESTL_UBATCH_WAVES, ESTL_UBATCH_WAVES)))
+#endif
+k_ubatch(UArgs a) {
     __shared__ float2 lds[kULds];
     __shared__ int s_item;
 #pragma unroll 1
