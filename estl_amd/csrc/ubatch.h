@@ -5,8 +5,15 @@
 
 namespace estl {
 
-constexpr int kUMaxLog2 = 20;       // largest transform of an uneven batch
-constexpr int kUMaxSmallLog2 = 12;  // up to here a work item is a group of rows
+// work-queue workgroup size 2^kUThreadsLog2 (see workqueue.hip)
+#ifndef ESTL_UBATCH_TLOG
+#define ESTL_UBATCH_TLOG 9
+#endif
+constexpr int kUThreadsLog2 = ESTL_UBATCH_TLOG;
+constexpr int kUMaxLog2 = 20;  // largest transform of an uneven batch
+// up to here a work item is a group of rows (a large item must fill a workgroup's tile:
+// 2^(kUThreadsLog2 + 4) values)
+constexpr int kUMaxSmallLog2 = kUThreadsLog2 + 3 > 12 ? kUThreadsLog2 + 3 : 12;
 
 // One work item: `count` transforms of size 2^log2n, the `first`-th.. of the
 // size-sorted offset list (count = 1 for large sizes).
@@ -19,7 +26,7 @@ struct UItem {
 
 // Device tables of every size class.
 struct UTables {
-    const float2 *block[kUMaxSmallLog2 + 1];  // BlockGeom tables, log2 0..12
+    const float2 *block[kUMaxSmallLog2 + 1];  // BlockGeom tables, log2 0..kUMaxSmallLog2
     const float2 *lo[kUMaxLog2 + 1];          // pre-twiddle tables of the second pass
     const float2 *hi[kUMaxLog2 + 1];
     int lobits[kUMaxLog2 + 1];

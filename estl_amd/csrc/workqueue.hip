@@ -24,10 +24,6 @@ namespace estl {
 // items tiles of 2^(kUThreadsLog2 + 4 - B) columns.  Measured on MI355X (8192 sizes
 // 2^U{8..18}): 512 threads at four waves per SIMD 2.53 ms, 256 threads 3.42 ms (its
 // 8-column tiles of radix 2^9 read 64-byte runs), 512 threads at three waves 2.97 ms.
-#ifndef ESTL_UBATCH_TLOG
-#define ESTL_UBATCH_TLOG 9
-#endif
-constexpr int kUThreadsLog2 = ESTL_UBATCH_TLOG;
 constexpr int kUThreads = 1 << kUThreadsLog2;
 static_assert(kUThreadsLog2 + 4 <= kUMaxSmallLog2 + 1, "every large size must fill a workgroup's tile");
 
@@ -68,7 +64,7 @@ constexpr int large_lds_max(std::integer_sequence<int, Ls...>) {
 constexpr int kULdsSmall = small_lds_max(std::make_integer_sequence<int, kUMaxSmallLog2>{});
 constexpr int kULdsLarge = large_lds_max(std::make_integer_sequence<int, kUMaxLog2 - kUMaxSmallLog2>{});
 constexpr int kULds = kULdsSmall > kULdsLarge ? kULdsSmall : kULdsLarge;
-static_assert(kULds * 8 <= 80 * 1024, "work-queue LDS must leave room for two workgroups per CU");
+static_assert(kULds * 8 <= (kUThreadsLog2 >= 10 ? 160 : 80) * 1024, "work-queue LDS over budget");
 
 template <int L, int DIR>
 __device__ __forceinline__ void small_item(const UArgs &a, const UItem &it, float2 *lds) {
@@ -155,7 +151,13 @@ k_ubatch(UArgs a) {
 #define LARGE(L) \
     case L: large_item<L, DIR>(a, it, lds); break;
             SMALL(1) SMALL(2) SMALL(3) SMALL(4) SMALL(5) SMALL(6) SMALL(7) SMALL(8) SMALL(9) SMALL(10) SMALL(11)
-            SMALL(12) LARGE(13) LARGE(14) LARGE(15) LARGE(16) LARGE(17) LARGE(18) LARGE(19) LARGE(20)
+            SMALL(12)
+#if ESTL_UBATCH_TLOG >= 10
+            SMALL(13)
+#else
+            LARGE(13)
+#endif
+            LARGE(14) LARGE(15) LARGE(16) LARGE(17) LARGE(18) LARGE(19) LARGE(20)
 #undef SMALL
 #undef LARGE
             default: break;
@@ -184,6 +186,9 @@ int ubatch_small_rows_per_item(int log2n) {
 #define TPWC(L) \
     case L: return USmall<L>::TPW;
         TPWC(1) TPWC(2) TPWC(3) TPWC(4) TPWC(5) TPWC(6) TPWC(7) TPWC(8) TPWC(9) TPWC(10) TPWC(11) TPWC(12)
+#if ESTL_UBATCH_TLOG >= 10
+        TPWC(13)
+#endif
 #undef TPWC
         default: return 1;
     }
