@@ -225,3 +225,32 @@ def test_linearity(gpu):
     y1 = run(torch, p, x[1:2].contiguous())
     yc = run(torch, p, comb)
     assert oracle.rel_l2(yc.cpu().numpy(), (a * y0 + c * y1).cpu().numpy()) <= tol(n)
+
+
+@pytest.mark.parametrize("log2n", [28, 30])
+def test_maximum_sizes_tone_and_round_trip(gpu, log2n):
+    """The largest transforms (2^30 = 8 GiB per buffer; offsets inside a row are 32-bit):
+    a unit tone transforms to exactly N at its bin (rel-L2 over the whole spectrum from
+    Parseval: |y|^2 summed minus the peak), and backward/N returns the input."""
+    import torch
+    n = 1 << log2n
+    f = 987654321 % n
+    t = torch.arange(n, device=gpu, dtype=torch.int64)
+    ph = (2 * math.pi / n) * torch.remainder(t * f, n).to(torch.float64)
+    del t
+    x = torch.polar(torch.ones_like(ph), ph).to(torch.complex64).reshape(1, n)
+    del ph
+    y = torch.empty_like(x)
+    p = estl_amd.Plan(n, 1, estl_amd.FORWARD)
+    assert p.launches() >= 3
+    p.execute(x, y)
+    torch.cuda.synchronize()
+    peak = complex(y[0, f].item())
+    energy = (y.abs().double() ** 2).sum().item()
+    err = math.sqrt(max(energy - abs(peak) ** 2, 0.0) + abs(peak - n) ** 2) / n
+    assert err <= tol(n), err
+    b = estl_amd.Plan(n, 1, estl_amd.BACKWARD, flags=estl_amd.FLAG_NORMALIZE)
+    b.execute(y, y)                     # in place
+    torch.cuda.synchronize()
+    num = torch.linalg.vector_norm((y - x).to(torch.complex128)).item()
+    assert num / math.sqrt(n) <= 2 * tol(n)
