@@ -203,6 +203,7 @@ static void conv_free(Conv *c) {
     if (!c) return;
     for (float2 *p : {c->buf.spec, c->buf.hp, c->buf.dc, c->buf.scratch[0], c->buf.scratch[1]})
         if (p) (void)hipFree(p);
+    free_stream_scratch(&c->shape.spec_plan);
     for (auto &s : c->shape.spec_plan.scratch)
         if (s) (void)hipFree(s);
     delete c;

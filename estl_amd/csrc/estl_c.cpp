@@ -254,6 +254,16 @@ int ensure_scratch(Plan *p, int which) {
     return ESTL_OK;
 }
 
+int stream_scratch(Plan *p, hipStream_t st, bool two, float2 **ws0, float2 **ws1) {
+    std::lock_guard<std::mutex> lk(p->by_stream->mu);
+    auto &pr = p->by_stream->bufs[st];
+    if (!pr.first) HIP_TRY(hipMalloc(&pr.first, p->scratch_bytes));
+    if (two && !pr.second) HIP_TRY(hipMalloc(&pr.second, p->scratch_bytes));
+    *ws0 = pr.first;
+    if (ws1) *ws1 = pr.second;
+    return ESTL_OK;
+}
+
 int execute_plan(Plan *p, const float2 *in, float2 *out, hipStream_t st) {
     return execute_plan_ws(p, in, out, nullptr, nullptr, st);
 }
@@ -279,10 +289,10 @@ int execute_plan_ws(Plan *p, const float2 *in, float2 *out, float2 *ws0, float2 
         case Plan::PASSES: {
             const int P = (int)p->steps.size();
             int rc = ESTL_OK;
-            if (!ws0) {
-                rc = ensure_scratch(p, 0);
+            const bool own = !ws0;  // no caller workspace: this stream's scratch
+            if (own) {
+                rc = stream_scratch(p, st, false, &ws0, nullptr);
                 if (rc) return rc;
-                ws0 = p->scratch[0];
             }
             // destinations backwards from out, alternating with scratch 0; the first pass
             // of a longer chain must not write its own input, so an in-place odd-length
@@ -292,9 +302,10 @@ int execute_plan_ws(Plan *p, const float2 *in, float2 *out, float2 *ws0, float2 
             for (int q = P - 1, k = 0; q >= 0; --q, ++k) dst[q] = (k & 1) ? ws0 : out;
             if (P > 1 && dst[0] == in) {
                 if (!ws1) {
-                    rc = ensure_scratch(p, 1);
+                    if (!own) return set_error(ESTL_ERR_INVALID, "an in-place odd-length chain needs a second workspace");
+                    float2 *w0 = nullptr;
+                    rc = stream_scratch(p, st, true, &w0, &ws1);
                     if (rc) return rc;
-                    ws1 = p->scratch[1];
                 }
                 dst[0] = ws1;
             }
@@ -309,8 +320,19 @@ int execute_plan_ws(Plan *p, const float2 *in, float2 *out, float2 *ws0, float2 
     return set_error(ESTL_ERR_INVALID, "bad plan");
 }
 
+void free_stream_scratch(Plan *p) {
+    if (!p->by_stream) return;
+    std::lock_guard<std::mutex> lk(p->by_stream->mu);
+    for (auto &kv : p->by_stream->bufs) {
+        if (kv.second.first) (void)hipFree(kv.second.first);
+        if (kv.second.second) (void)hipFree(kv.second.second);
+    }
+    p->by_stream->bufs.clear();
+}
+
 void destroy_plan(Plan *p) {
     if (!p) return;
+    free_stream_scratch(p);
     for (auto &s : p->scratch)
         if (s) (void)hipFree(s);
     if (p->h_in_dev) (void)hipFree(p->h_in_dev);

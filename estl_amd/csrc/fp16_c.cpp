@@ -33,14 +33,15 @@ int execute_plan_fp16(Plan *p, const void *in, const float *in_scale, void *out,
             break;
         case Plan::PASSES: {
             const int P = (int)p->steps.size();
-            int rc = ensure_scratch(p, 0);
+            float2 *ws0 = nullptr, *ws1 = nullptr;  // this stream's scratch
+            int rc = stream_scratch(p, st, false, &ws0, nullptr);
             if (rc) return rc;
             std::vector<void *> dst(P);
-            for (int q = P - 1, k = 0; q >= 0; --q, ++k) dst[q] = (k & 1) ? (void *)p->scratch[0] : out;
+            for (int q = P - 1, k = 0; q >= 0; --q, ++k) dst[q] = (k & 1) ? (void *)ws0 : out;
             if (P > 1 && dst[0] == in) {  // a single pass may run in place
-                rc = ensure_scratch(p, 1);
+                rc = stream_scratch(p, st, true, &ws0, &ws1);
                 if (rc) return rc;
-                dst[0] = p->scratch[1];
+                dst[0] = ws1;
             }
             const void *src = in;
             for (int q = 0; q < P && e == hipSuccess; ++q) {

@@ -4,7 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "internal.h"
@@ -29,7 +33,14 @@ struct Plan {
     const float2 *tw_rows = nullptr;
     std::vector<PassStep> steps;
     size_t scratch_bytes = 0;
-    float2 *scratch[2] = {nullptr, nullptr};
+    float2 *scratch[2] = {nullptr, nullptr};  // the plan's own (profiling, synchronous paths)
+    // scratch of executes without caller workspace, one pair per stream, so executes of
+    // one plan on different streams never share a buffer
+    struct StreamScratch {
+        std::mutex mu;
+        std::map<hipStream_t, std::pair<float2 *, float2 *>> bufs;
+    };
+    std::shared_ptr<StreamScratch> by_stream = std::make_shared<StreamScratch>();
     // resident buffers and stream of the synchronous host entry point
     float2 *h_in_dev = nullptr, *h_out_dev = nullptr;
     hipStream_t stream = nullptr;
@@ -67,7 +78,10 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
 int execute_plan(Plan *p, const float2 *in, float2 *out, hipStream_t st);
 int execute_plan_ws(Plan *p, const float2 *in, float2 *out, float2 *ws0, float2 *ws1, hipStream_t st);
 int ensure_scratch(Plan *p, int which);
+// scratch pair of stream st (allocated on first use; second only if `two`)
+int stream_scratch(Plan *p, hipStream_t st, bool two, float2 **ws0, float2 **ws1);
 void destroy_plan(Plan *p);
+void free_stream_scratch(Plan *p);  // the per-stream scratch of a plan not destroyed by destroy_plan
 std::string describe_plan(const Plan *p);
 int plan_launches(const Plan *p);
 int tune_plan(long n, long batch, int dir, int precision, bool *rows, std::vector<int> *split, std::vector<int> *tiles,

@@ -99,9 +99,11 @@ static int execute_plan2d(Plan2D *p, const float2 *in, float2 *out, hipStream_t 
 
 static void destroy_plan2d(Plan2D *p) {
     if (!p) return;
-    for (Plan *q : {&p->rows, &p->cols})
+    for (Plan *q : {&p->rows, &p->cols}) {
+        free_stream_scratch(q);
         for (auto &s : q->scratch)
             if (s) (void)hipFree(s);
+    }
     if (p->h_in_dev) (void)hipFree(p->h_in_dev);
     if (p->h_out_dev) (void)hipFree(p->h_out_dev);
     if (p->stream) (void)hipStreamDestroy(p->stream);

@@ -171,6 +171,7 @@ static int time_candidate(const Choice &c, long n, long batch, int dir, int prec
     for (auto &s : p.scratch)
         if (s) (void)hipFree(s);
     p.scratch[0] = p.scratch[1] = nullptr;
+    free_stream_scratch(&p);
     *us = best;
     return rc;
 }

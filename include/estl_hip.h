@@ -193,7 +193,9 @@ int estl_fill_uniform(void *d_dst, long count, uint64_t seed, uint64_t offset, v
  * out buffers (offsets NULL: packed back to back).  No reference counterpart: the
  * reference batches only equal sizes (batchCount, FFT.h:25-27) and needs one FFT object,
  * program and launch sequence per size (tests/Test.cpp:48-53).  Transforms must not
- * overlap; the buffers must hold the extent estl_ubatch_describe reports. */
+ * overlap; the buffers must hold the extent estl_ubatch_describe reports.  One object
+ * holds one work-queue ticket and one scratch region: executes of the same object must
+ * not overlap in time (order them on one stream); separate objects run concurrently. */
 int estl_ubatch_create(estl_ubatch_t *ub, int count, const long *sizes, const long *offsets, int direction,
                        unsigned flags);
 

@@ -254,3 +254,28 @@ def test_maximum_sizes_tone_and_round_trip(gpu, log2n):
     torch.cuda.synchronize()
     num = torch.linalg.vector_norm((y - x).to(torch.complex128)).item()
     assert num / math.sqrt(n) <= 2 * tol(n)
+
+
+def test_one_plan_on_concurrent_streams(gpu):
+    """A multi-pass plan executed at once on several streams (each stream gets its own
+    scratch): every result equals the same plan run alone, bit for bit."""
+    import torch
+    n, batch = 1 << 20, 4
+    p = estl_amd.Plan(n, batch, estl_amd.FORWARD)
+    assert p.launches() >= 2
+    xs = [dev_uniform(torch, batch, n, seed=50 + i, device=gpu) for i in range(4)]
+    refs = []
+    for x in xs:
+        y = torch.empty_like(x)
+        p.execute(x, y)
+        torch.cuda.synchronize()
+        refs.append(y)
+    streams = [torch.cuda.Stream() for _ in xs]
+    outs = [torch.empty_like(x) for x in xs]
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for x, y, s in zip(xs, outs, streams):
+            p.execute(x, y, s)
+        torch.cuda.synchronize()
+        for y, r in zip(outs, refs):
+            assert torch.equal(y, r), rep
