@@ -145,7 +145,8 @@ int estl_execute_fp16(estl_plan_t plan, const void *d_in, const float *d_in_scal
 
 /* Synchronous execute on host buffers with the reference's semantics: copy in, run,
  * copy out, wait (FFT_OpenCL_Contiguous::execute, FFT_OpenCL_Contiguous.h:97-142, and
- * FFT_FFTW3::execute, FFT_FFTW3.h:31).  The plan keeps its device buffers resident. */
+ * FFT_FFTW3::execute, FFT_FFTW3.h:31).  The plan keeps its device buffers resident, so
+ * host executes of one plan from several threads must be serialised by the caller. */
 int estl_execute_host(estl_plan_t plan, const void *h_in, void *h_out);
 
 /* Largest batch of length-n rows whose working set fits `budget_bytes` of device memory
@@ -222,7 +223,9 @@ int estl_conv_create(estl_conv_t *conv, long n, long rows, int precision, unsign
  * spectrum resident on the device; later executes reuse it. */
 int estl_conv_set_filter(estl_conv_t conv, const float *d_h, long k, void *stream);
 
-/* y = x (*) h for all rows (device buffers of rows * n floats; asynchronous). */
+/* y = x (*) h for all rows (device buffers of rows * n floats; asynchronous).  The
+ * object's spectrum and scratch are its own: executes (and set_filter) of one object
+ * must not overlap in time; the server (estl_server_*) gives every request its own. */
 int estl_conv_execute(estl_conv_t conv, const float *d_x, float *d_y, void *stream);
 
 /* FP16-storage convolution (object created with ESTL_PRECISION_FP16; no counterpart in
