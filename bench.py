@@ -503,6 +503,23 @@ def main() -> int:
     rt = (torch.linalg.vector_norm((z / N - x).to(torch.complex128)) /
           torch.linalg.vector_norm(x.to(torch.complex128))).item()
 
+    # the runtime's device-to-device copy of the same bytes (512 MiB read + 512 MiB
+    # written, hipMemcpyAsync through torch), timed the same way: a plain read-once /
+    # write-once reference beside the transform
+    w = torch.empty_like(x)
+    cev = []
+    for _ in range(10):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(stream):
+            a.record(stream)
+            w.copy_(x)
+            b.record(stream)
+        cev.append((a, b))
+    stream.synchronize()
+    copy_ms = sorted(a.elapsed_time(b) for a, b in cev)[5]
+    copy_gbs = launch_bytes / (copy_ms * 1e-3) / 1e9
+    del w
+
     # per-call form: one launch per transform, sequential, same stream, device-resident
     prow = min(args.percall_rows, BATCH)
     f1 = estl_amd.Plan(N, 1, estl_amd.FORWARD)
@@ -571,7 +588,9 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(),
                      "kernel": "k_fft_rows<12,-1,1> (forward, N=4096)",
-                     "launch_ms_mean": round(fwd_mean, 5), "backward_launch_ms_mean": round(bwd_mean, 5)},
+                     "launch_ms_mean": round(fwd_mean, 5), "backward_launch_ms_mean": round(bwd_mean, 5),
+                     "device_copy_gbs": round(copy_gbs, 1),
+                     "over_device_copy": round(achieved / copy_gbs, 3)},
         "per_call": {"value": round(percall_gbs, 2), "unit": "GB/s", "rows": prow,
                      "launches": 2 * prow, "batched_over_per_call": round(value / percall_gbs, 1)},
         "latency_us": {"step_p50": round(1e3 * percentile(step_ms, 50), 2),
