@@ -5,11 +5,13 @@
 // host round trip per row and column, FFT.h:65-81), multiplies on the host
 // (Convolution.h:125-137), transforms back and scales by 1/(K^[0] W H)
 // (Convolution.h:141-143).  Here:
-//   * k_conv_fold   builds the length-n filter: h folded modulo n (a filter longer than
-//                   the signal wraps, the circular definition), optionally rotated so
-//                   its centre (K + 1)/2 lands on index 0 (the reference's kernel shift,
-//                   Convolution.h:113-120); the spectrum is then one forward FFT, kept
-//                   resident;
+//   * the length-n filter is h folded modulo n (a filter longer than the signal
+//                   wraps, the circular definition), optionally rotated so its centre
+//                   (K + 1)/2 lands on index 0 (the reference's kernel shift,
+//                   Convolution.h:113-120), built by the spectrum transform as it loads
+//                   the taps (fold_tap): for n <= 2^13 k_filter_spectrum (fold, forward
+//                   FFT and the kernel-sum normalisation in one workgroup), above that
+//                   the first pass of the spectrum plan; the spectrum stays resident;
 //   * k_conv_rows   for n <= 2^13: two real rows enter as one complex row (re = row 2p,
 //                   im = row 2p + 1 -- the filter is real, so the two convolutions stay
 //                   separate), forward FFT, multiply by the spectrum (its conjugate for
@@ -22,16 +24,6 @@
 #include "internal.h"
 
 namespace estl {
-
-__global__ void k_conv_fold(const float *__restrict__ h, long k, long n, long shift, float2 *__restrict__ hp) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    // hp[i] = sum of h[t] over t with (t - shift) = i (mod n)
-    double acc = 0.0;
-    long t = (i + shift) % n;
-    for (; t < k; t += n) acc += (double)h[t];
-    hp[i] = make_float2((float)acc, 0.0f);
-}
 
 // spectrum /= Re spectrum[0] (the kernel sum), read from a copy `dc` taken beforehand
 __global__ void k_spec_normalize(float2 *__restrict__ spec, long n, const float2 *__restrict__ dc) {
@@ -46,9 +38,45 @@ hipError_t launch_spec_normalize(float2 *spec, long n, const float2 *dc, hipStre
     return hipGetLastError();
 }
 
-hipError_t launch_conv_fold(const float *h, long k, long n, long shift, float2 *hp, hipStream_t st) {
-    const long blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(k_conv_fold, dim3((unsigned)blocks), dim3(256), 0, st, h, k, n, shift, hp);
+// The spectrum of a filter of length n <= 2^13 in one workgroup: fold the taps on
+// load, forward FFT (the rows kernel's block_fft and table, so the bins are those of the
+// rows plan), optionally divide by the kernel sum Re K^[0] (Convolution.h:141), store.
+template <int LOG2N>
+__global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT)
+k_filter_spectrum(const float *__restrict__ h, long k, long shift, float2 *__restrict__ spec,
+                  const float2 *__restrict__ tw, int normalize) {
+    using G = BlockGeom<LOG2N>;
+    constexpr int N = G::N, E = G::E, TPT = G::TPT;
+    __shared__ float2 lds[G::uses_lds ? Lds<kLdsPad>::size(N) : 1];
+    __shared__ float s_dc;
+    const int tl = opaque_lane(threadIdx.x);
+    float2 v[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = make_float2(fold_tap(h, k, N, shift, tl + TPT * i), 0.0f);
+    block_fft<LOG2N, -1, kLdsPad>(v, lds, tl, tw, false);
+    if (normalize) {
+        if (tl == 0) s_dc = v[0].x;  // bin 0
+        __syncthreads();
+        const float s = 1.0f / s_dc;
+#pragma unroll
+        for (int i = 0; i < E; ++i) v[i] = make_float2(v[i].x * s, v[i].y * s);
+    }
+#pragma unroll
+    for (int i = 0; i < E; ++i) spec[tl + TPT * i] = v[i];
+}
+
+hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift, float2 *spec, const float2 *tw,
+                                  bool normalize, hipStream_t st) {
+    switch (log2n) {
+#define FS(L)                                                                                                  \
+    case L:                                                                                                    \
+        hipLaunchKernelGGL(k_filter_spectrum<L>, dim3(1), dim3(BlockGeom<L>::TPT), 0, st, h, k, shift, spec, tw, \
+                           normalize ? 1 : 0);                                                                 \
+        break;
+        FS(1) FS(2) FS(3) FS(4) FS(5) FS(6) FS(7) FS(8) FS(9) FS(10) FS(11) FS(12) FS(13)
+#undef FS
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

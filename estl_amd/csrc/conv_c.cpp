@@ -25,13 +25,24 @@ namespace estl {
 
 hipError_t launch_conv_rows(int log2n, bool corr, const float *x, float *y, long rows, const float2 *spec,
                             const float2 *tw, float scale, hipStream_t st);
-hipError_t launch_conv_fold(const float *h, long k, long n, long shift, float2 *hp, hipStream_t st);
+hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift, float2 *spec, const float2 *tw,
+                                  bool normalize, hipStream_t st);
 hipError_t launch_spec_normalize(float2 *spec, long n, const float2 *dc, hipStream_t st);
 hipError_t launch_conv_rows_fp16(int log2n, bool corr, const void *x, void *y, long rows, const float2 *spec,
                                  const float2 *tw, float scale, hipStream_t st);
 hipError_t launch_conv_fp16_filter(const float *h, long k, float2 *spec, long n, const float2 *dc, bool normalized,
                                    hipStream_t st);
 hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1, long rows, hipStream_t st);
+
+int group_spectrum_plan(Plan *p, long n, long filters) {
+    int rc = build_plan(p, n, filters, ESTL_FORWARD, ESTL_PRECISION_FP32, 0, nullptr);
+    if (rc) return rc;
+    if (p->kind != Plan::PASSES) return set_error(ESTL_ERR_UNSUPPORTED, "group spectra need a multi-pass size");
+    PassStep &s0 = p->steps.front();
+    while ((1 << s0.log2t) > (1L << s0.args.log2S) && s0.log2t > 0) --s0.log2t;
+    s0.args.io_in = 3;
+    return ESTL_OK;
+}
 
 int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precision) {
     if (n < 2 || (n & (n - 1))) return set_error(ESTL_ERR_INVALID, "n=%ld is not a power of two >= 2", n);
@@ -53,6 +64,14 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precisi
     c->spec_bytes = sizeof(float2) * (size_t)(h ? n + 1 : n);
     int rc = build_plan(&c->spec_plan, n, 1, ESTL_FORWARD, ESTL_PRECISION_FP32, 0, nullptr);
     if (rc) return rc;
+    if (c->spec_plan.kind == Plan::PASSES) {
+        // the first pass builds the filter from its taps through its direct load path
+        PassStep &s0 = c->spec_plan.steps.front();
+        while ((1 << s0.log2t) > (1L << s0.args.log2S) && s0.log2t > 0) --s0.log2t;
+        s0.args.io_in = 3;
+    } else if (c->spec_plan.kind != Plan::ROWS) {
+        return set_error(ESTL_ERR_UNSUPPORTED, "convolution of n=%ld: no spectrum plan", n);
+    }
     if (c->log2n <= kMaxRowsLog2) {
         c->fused = true;
         c->scratch_bytes = 0;
@@ -95,19 +114,41 @@ size_t conv_scratch_bytes(const ConvShape &c) {
 }
 
 size_t conv_footprint(const ConvShape &c) {
-    // spectrum + folded filter + dc + two scratch buffers (the spectrum FFT of a
-    // multi-pass size borrows the scratch)
-    return 2 * c.spec_bytes + 256 + 2 * conv_scratch_bytes(c);
+    // spectrum + dc + two scratch buffers (the spectrum FFT of a multi-pass size borrows
+    // the first)
+    return c.spec_bytes + 256 + 2 * conv_scratch_bytes(c);
 }
 
 int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k, hipStream_t st) {
     if (!h || k < 1) return set_error(ESTL_ERR_INVALID, "filter needs k >= 1 values");
     const long shift = (c.flags & ESTL_CONV_CENTER) ? ((k + 1) / 2) % c.n : 0;
-    hipError_t e = launch_conv_fold(h, k, c.n, shift, b.hp, st);
-    if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "fold launch: %s", hipGetErrorString(e));
-    int rc = execute_plan_ws(const_cast<Plan *>(&c.spec_plan), b.hp, b.spec, b.scratch[0], b.scratch[1], st);
-    if (rc) return rc;
-    if (c.flags & ESTL_CONV_NORMALIZE_KERNEL) {
+    const bool normalize = (c.flags & ESTL_CONV_NORMALIZE_KERNEL) != 0;
+    hipError_t e = hipSuccess;
+    if (c.spec_plan.kind == Plan::ROWS) {
+        // fold, transform and normalise in one launch
+        e = launch_filter_spectrum(c.log2n, h, k, shift, b.spec, c.spec_plan.tw_rows, normalize, st);
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "filter spectrum launch: %s", hipGetErrorString(e));
+    } else {
+        // the spectrum plan's passes, the first building the filter from its taps; the
+        // destinations alternate back from the spectrum with scratch 0
+        if (!b.scratch[0]) return set_error(ESTL_ERR_INVALID, "the filter spectrum of n=%ld needs scratch", c.n);
+        const auto &steps = c.spec_plan.steps;
+        const int P = (int)steps.size();
+        const float2 *src = b.scratch[0];  // not read by the first pass (it reads h)
+        for (int q = 0; q < P; ++q) {
+            PassArgs a = steps[q].args;
+            if (q == 0) {
+                a.rin = h;
+                a.fk = k;
+                a.fshift = shift;
+            }
+            float2 *dst = ((P - 1 - q) & 1) ? b.scratch[0] : b.spec;
+            e = launch_pass(steps[q].log2b, steps[q].log2t, ESTL_FORWARD, src, dst, a, st);
+            if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "filter spectrum pass: %s", hipGetErrorString(e));
+            src = dst;
+        }
+    }
+    if (normalize && c.spec_plan.kind != Plan::ROWS) {
         // divide by the kernel sum K^[0] (Convolution.h:141): spectrum / spectrum[0].re
         e = hipMemcpyAsync(b.dc, b.spec, sizeof(float2), hipMemcpyDeviceToDevice, st);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "spectrum dc copy: %s", hipGetErrorString(e));
@@ -145,6 +186,54 @@ int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y,
         hipError_t e = dir ? launch_pass(s.log2b, s.log2t, dir, src, dst, a, st)
                            : launch_pass_mid(s.log2b, s.log2t, src, dst, a, st);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "conv pass launch: %s", hipGetErrorString(e));
+        src = dst;
+        cur ^= 1;
+        return ESTL_OK;
+    };
+    for (const auto &s : c.fwd)
+        if (int rc = run(s, ESTL_FORWARD)) return rc;
+    if (int rc = run(c.mid, 0)) return rc;
+    for (const auto &s : c.inv)
+        if (int rc = run(s, ESTL_BACKWARD)) return rc;
+    return ESTL_OK;
+}
+
+// A request group (ConvGroup in device memory at d_group).  Spectra: the G filters as
+// the G rows of `spec` (a spectrum plan over G rows whose first pass builds them from
+// their taps), destinations alternating back from `spectra` with scratch0.
+int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectra, float2 *scratch0, hipStream_t st) {
+    if (spec.kind != Plan::PASSES || spec.steps.empty() || spec.steps.front().args.io_in != 3)
+        return set_error(ESTL_ERR_INVALID, "not a group spectrum plan");
+    const int P = (int)spec.steps.size();
+    const float2 *src = scratch0;  // not read by the first pass (it reads the taps)
+    for (int q = 0; q < P; ++q) {
+        PassArgs a = spec.steps[q].args;
+        a.group = d_group;
+        float2 *dst = ((P - 1 - q) & 1) ? scratch0 : spectra;
+        hipError_t e = launch_pass(spec.steps[q].log2b, spec.steps[q].log2t, ESTL_FORWARD, src, dst, a, st);
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group spectrum pass: %s", hipGetErrorString(e));
+        src = dst;
+    }
+    return ESTL_OK;
+}
+
+// The convolution passes of a request group over the shape's pairs (the members' pairs
+// back to back): the first pass reads each member's rows, the middle pass multiplies by
+// each member's spectrum, the last writes each member's rows.
+int conv_run_group(const ConvShape &c, const ConvGroup *d_group, float2 *scratch0, float2 *scratch1, hipStream_t st) {
+    if (c.fused || c.precision != ESTL_PRECISION_FP32)
+        return set_error(ESTL_ERR_INVALID, "request groups run FP32 multi-pass convolutions");
+    float2 *scr[2] = {scratch0, scratch1};
+    int cur = 0;
+    const float2 *src = scratch1;  // not read by the first pass (it reads the members' rows)
+    auto run = [&](const PassStep &s, int dir) -> int {
+        PassArgs a = s.args;
+        a.group = d_group;
+        if (a.io_out == 2) a.scale = c.scale;
+        float2 *dst = scr[cur];
+        hipError_t e = dir ? launch_pass(s.log2b, s.log2t, dir, src, dst, a, st)
+                           : launch_pass_mid(s.log2b, s.log2t, src, dst, a, st);
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group conv pass launch: %s", hipGetErrorString(e));
         src = dst;
         cur ^= 1;
         return ESTL_OK;
@@ -201,7 +290,7 @@ struct Conv {
 
 static void conv_free(Conv *c) {
     if (!c) return;
-    for (float2 *p : {c->buf.spec, c->buf.hp, c->buf.dc, c->buf.scratch[0], c->buf.scratch[1]})
+    for (float2 *p : {c->buf.spec, c->buf.dc, c->buf.scratch[0], c->buf.scratch[1]})
         if (p) (void)hipFree(p);
     free_stream_scratch(&c->shape.spec_plan);
     for (auto &s : c->shape.spec_plan.scratch)
@@ -225,8 +314,7 @@ int estl_conv_create(estl_conv_t *conv, long n, long rows, int precision, unsign
         return rc;
     }
     const ConvShape &s = c->shape;
-    bool ok = hipMalloc(&c->buf.spec, s.spec_bytes) == hipSuccess && hipMalloc(&c->buf.hp, s.spec_bytes) == hipSuccess &&
-              hipMalloc(&c->buf.dc, sizeof(float2)) == hipSuccess;
+    bool ok = hipMalloc(&c->buf.spec, s.spec_bytes) == hipSuccess && hipMalloc(&c->buf.dc, sizeof(float2)) == hipSuccess;
     const size_t sb = conv_scratch_bytes(s);
     for (auto &p : c->buf.scratch)
         if (ok && sb) ok = hipMalloc(&p, sb) == hipSuccess;

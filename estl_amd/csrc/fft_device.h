@@ -92,6 +92,15 @@ __device__ __forceinline__ const T *opaque(const T *p) {
 // Same for a per-lane value (the thread index): in loops whose body dispatches on a
 // size class, addresses derived from the thread index alone would otherwise be hoisted
 // for every class at once and all stay live.
+// Value e of a convolution filter of length n built from k real taps h: the taps folded
+// modulo n (a filter longer than the signal wraps, the circular definition) and rotated
+// by `shift` -- the sum of h[t] over t = e + shift (mod n), accumulated in FP64.
+__device__ __forceinline__ float fold_tap(const float *__restrict__ h, long k, long n, long shift, long e) {
+    double acc = 0.0;
+    for (long t = (e + shift) % n; t < k; t += n) acc += (double)h[t];
+    return (float)acc;
+}
+
 __device__ __forceinline__ int opaque_lane(int x) {
     asm volatile("" : "+v"(x));
     return x;

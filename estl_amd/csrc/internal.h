@@ -7,6 +7,25 @@
 namespace estl {
 
 // Arguments of one global Stockham pass (k_fft_pass).
+// A group of convolution requests of one length n run as one batch (the server's
+// continuous batching): request q owns pair rows [pair_start[q], pair_start[q + 1]) of
+// the batch.  Its real rows are read from x[q] and written to y[q] (real_rows[q] of
+// them), its spectrum is spec[q]; in the group's spectrum passes row q is the filter of
+// request q (taps h[q], k[q] of them, rotation shift[q]).  Lives in device memory.
+constexpr int kGroupMax = 16;
+struct ConvGroup {
+    int count;
+    int pad_;
+    long pair_start[kGroupMax + 1];
+    long real_rows[kGroupMax];
+    const float *x[kGroupMax];
+    float *y[kGroupMax];
+    const float2 *spec[kGroupMax];
+    const float *h[kGroupMax];
+    long k[kGroupMax];
+    long shift[kGroupMax];
+};
+
 struct PassArgs {
     long batch;              // rows
     long A;                  // A_q: product of the radices of the earlier passes
@@ -23,6 +42,9 @@ struct PassArgs {
     //   spectrum / corr: the convolution middle pass (launch_pass_mid) multiplies its
     //   natural-order bins by spectrum[k] (its conjugate if corr)
     //   io_out = 2: write re / im of the scaled result to rout rows 2r, 2r + 1
+    //   io_in  = 3: the row is a convolution filter built from its fk real taps at rin:
+    //               folded modulo n and rotated by fshift (fold_tap); first pass of the
+    //               filter's spectrum plan, direct load path (S >= T)
     int io_in, io_out, corr;
     long real_rows;          // real rows behind the pairs (the last pair may be single)
     const float *rin;
@@ -32,6 +54,10 @@ struct PassArgs {
     int planar_in;           // `in` / `out` use the tile-planarized layout (FP32 only)
     int planar_out;
     const float2 *spectrum;
+    long fk, fshift;         // io_in = 3: taps and rotation of the filter
+    // FP32 convolution passes of a request group: rows index the group's batch, and the
+    // real-pair I/O, the filter loads and the middle pass's spectrum are the member's
+    const ConvGroup *group;
 };
 
 hipError_t launch_rows(int log2n, int dir, const float2 *in, float2 *out, long batch, const float2 *tw, float scale,

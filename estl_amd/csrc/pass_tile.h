@@ -37,6 +37,13 @@ struct PassTile {
     static constexpr int threads = BlockGeom<LOG2B>::TPT << LOG2T;
 };
 
+// The member of a request group that owns pair row `row` (uniform across the tile).
+__device__ __forceinline__ int group_member(const ConvGroup *g, long row) {
+    int q = 0;
+    while (q + 1 < g->count && g->pair_start[q + 1] <= row) ++q;
+    return q;
+}
+
 // Geometry of tile `tile` for the calling thread: the row, the tile's first column j0,
 // the thread's column j = j0 + c, its g = j / S and z = j mod S, and tl.
 template <int LOG2B, int LOG2T>
@@ -71,11 +78,19 @@ __device__ __forceinline__ void pass_tile_load(const ST *__restrict__ in, const 
     if (a.io_in == 1) {
         // real pair: rows 2 row and 2 row + 1 of rin (the second absent past the end),
         // FP32 or, with FP16 storage, binary16 reals; only the first pass reads pairs
-        // and its plans keep S >= T
-        const bool two = 2 * P.row + 1 < a.real_rows;
+        // and its plans keep S >= T.  In a request group: the member's pair and rows.
+        long prow = P.row, rrows = a.real_rows;
+        const float *xb = a.rin;
+        if (F32 && a.group) {
+            const int q = group_member(a.group, P.row);
+            prow = P.row - a.group->pair_start[q];
+            rrows = a.group->real_rows[q];
+            xb = a.group->x[q];
+        }
+        const bool two = 2 * prow + 1 < rrows;
         const int base = (P.g << (ls + LOG2B)) + P.z;
         if constexpr (F32) {
-            const float *x1 = a.rin + ((2 * P.row) << a.log2n);
+            const float *x1 = xb + ((2 * prow) << a.log2n);
 #pragma unroll
             for (int i = 0; i < E; ++i) {
                 const int o = base + ((P.tl + TPT * i) << ls);
@@ -89,6 +104,21 @@ __device__ __forceinline__ void pass_tile_load(const ST *__restrict__ in, const 
                 v[i] = make_float2((float)x1[o], two ? (float)x1[(1L << a.log2n) + o] : 0.0f);
             }
         }
+    } else if (F32 && a.io_in == 3) {
+        // the filter, built from its taps as it is loaded; its plans keep S >= T.  In a
+        // request group, row q is member q's filter
+        const long n = 1L << a.log2n;
+        const int base = (P.g << (ls + LOG2B)) + P.z;
+        const float *h = a.rin;
+        long fk = a.fk, fsh = a.fshift;
+        if (a.group) {
+            h = a.group->h[P.row];
+            fk = a.group->k[P.row];
+            fsh = a.group->shift[P.row];
+        }
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            v[i] = make_float2(fold_tap(h, fk, n, fsh, base + ((P.tl + TPT * i) << ls)), 0.0f);
     } else if (F32 && a.planar_in) {
         // the same values from the tile-planarized layout, by element index
         const long e0 = (P.row << a.log2n) +
@@ -135,7 +165,7 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
     const int lcpr = a.log2n - LOG2B;
     const int c = P.c, tl = P.tl, j = P.j;
     bool busy = lds_busy;
-    if (a.io_in != 1 && ls < LOG2T) {
+    if (a.io_in != 1 && a.io_in != 3 && ls < LOG2T) {
         // value o of the contiguous block belongs to column (o / (S B)) S + o mod S,
         // element (o mod S B) / S
         const int sbmask = (1 << (ls + LOG2B)) - 1, smask = (1 << ls) - 1;
@@ -179,10 +209,19 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
     }
     block_fft<LOG2B, DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, busy);
     if (a.io_out == 2) {
-        // re / im to real rows 2 row and 2 row + 1 of rout (FP32 or binary16)
-        const bool two = 2 * P.row + 1 < a.real_rows;
+        // re / im to real rows 2 row and 2 row + 1 of rout (FP32 or binary16); in a
+        // request group, the member's pair and rows
+        long prow = P.row, rrows = a.real_rows;
+        float *yb = a.rout;
+        if (F32 && a.group) {
+            const int q = group_member(a.group, P.row);
+            prow = P.row - a.group->pair_start[q];
+            rrows = a.group->real_rows[q];
+            yb = a.group->y[q];
+        }
+        const bool two = 2 * prow + 1 < rrows;
         if constexpr (F32) {
-            float *y1 = a.rout + ((2 * P.row) << a.log2n) + j;
+            float *y1 = yb + ((2 * prow) << a.log2n) + j;
 #pragma unroll
             for (int i = 0; i < E; ++i) {
                 const int o = (tl + TPT * i) << lcpr;
@@ -202,9 +241,10 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
     }
     if constexpr (MID) {
         // the last forward pass yields natural order: output h of column j is bin j + h n/B
+        const float2 *spec = (F32 && a.group) ? a.group->spec[group_member(a.group, P.row)] : a.spectrum;
 #pragma unroll
         for (int i = 0; i < E; ++i) {
-            const float2 hk = a.spectrum[j + ((tl + TPT * i) << lcpr)];
+            const float2 hk = spec[j + ((tl + TPT * i) << lcpr)];
             v[i] = a.corr ? cmulc(v[i], hk) : cmul(v[i], hk);
         }
         block_fft<LOG2B, -DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, true);

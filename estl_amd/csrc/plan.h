@@ -104,7 +104,7 @@ struct ConvShape {
                                      // last holding the filter's L1 bound)
 };
 struct ConvBuffers {
-    float2 *spec = nullptr, *hp = nullptr, *dc = nullptr;
+    float2 *spec = nullptr, *dc = nullptr;
     float2 *scratch[2] = {nullptr, nullptr};
 };
 int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precision = 0);
@@ -113,6 +113,11 @@ size_t conv_footprint(const ConvShape &c);  // bytes of ConvBuffers
 // (FP32) spectrum transform, which borrows scratch[0], needs more
 size_t conv_scratch_bytes(const ConvShape &c);
 int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k, hipStream_t st);
+// request groups (the server's continuous batching): the spectrum plan over `filters`
+// rows, the group's spectra and its convolution passes
+int group_spectrum_plan(Plan *p, long n, long filters);
+int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectra, float2 *scratch0, hipStream_t st);
+int conv_run_group(const ConvShape &c, const ConvGroup *d_group, float2 *scratch0, float2 *scratch1, hipStream_t st);
 int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y, hipStream_t st);
 // FP16 storage: x16 / y16 binary16 rows, y = (stored y) * s_out[row]
 int conv_run_fp16(const ConvShape &c, const ConvBuffers &b, const void *x16, const float *s_in, void *y16,

@@ -12,11 +12,18 @@
 //   * continuous batching: admitted requests are launched at once on a pool of streams
 //     and run concurrently with the ones already on the device; their memory is
 //     returned the moment they complete, admitting the next;
+//   * request groups: a multi-pass FP32 convolution at the head of the queue takes
+//     along the queued requests of the same kind, length and flags found in the next
+//     kGroupWindow entries (up to kGroupMax requests, kGroupMaxValues values), and the
+//     group runs as one batch: one launch per pass for all of them (the pass kernels
+//     find each row's request in a ConvGroup descriptor) instead of one chain of
+//     launches per request.  Each member keeps its own output, id and release;
 //   * resident state: plans, twiddle tables and convolution shapes are cached per size
 //     and stay on the device; a filter can be registered once (its spectrum then lives
 //     in the arena, counted against the budget) and used by any number of requests;
-//   * the same plan and kernels run for a request whatever runs beside it, and no
-//     value crosses requests, so each result is bit-identical alone and in the mix;
+//   * the same pass split and kernels run for a request whatever runs beside it (a
+//     group only changes which columns share a tile, not what a column computes), and
+//     no value crosses requests, so each result is bit-identical alone and in the mix;
 //   * latency (submit -> completion observed) and throughput are recorded per request;
 //     completed ids are delivered through a completion queue.
 // One dispatcher thread admits requests in arrival order; one launcher thread per stream
@@ -136,7 +143,18 @@ struct Request {
     ConvShape *shape = nullptr;
     const float2 *spectrum = nullptr;
     int stream = 0;
+    // request group: the leader launches and completes it; followers ride along
+    std::vector<long long> followers;  // leader
+    long long leader = 0;              // follower
+    ConvShape *gshape = nullptr;       // leader: the shape over the group's pairs
+    Plan *gspec = nullptr;             // leader: spectrum plan over the group's filters
+    int gslot = -1;                    // leader: its pinned descriptor slot
 };
+
+// request groups
+constexpr int kGroupWindow = 256;            // queued requests searched for companions
+constexpr long kGroupMaxValues = 1L << 24;   // real values of a group
+constexpr int kGroupSlots = 64;              // pinned descriptor slots (> in-flight groups)
 
 class Server {
   public:
@@ -153,6 +171,9 @@ class Server {
             streams_.push_back(s);
         }
         max_inflight_ = 2 * nstreams_;
+        if (hipHostMalloc((void **)&host_desc_, sizeof(ConvGroup) * kGroupSlots) != hipSuccess)
+            return set_error(ESTL_ERR_NOMEM, "pinned group descriptors");
+        for (int i = kGroupSlots - 1; i >= 0; --i) free_slots_.push_back(i);
         work_.resize(nstreams_);
         cv_work_ = std::vector<std::condition_variable>(nstreams_);
         for (int i = 0; i < nstreams_; ++i) launchers_.emplace_back([this, i] { launch_loop(i); });
@@ -181,32 +202,44 @@ class Server {
             if (kv.second.ev_in) (void)hipEventDestroy(kv.second.ev_in);
         }
         for (auto e : events_) (void)hipEventDestroy(e);
+        if (host_desc_) (void)hipHostFree(host_desc_);
     }
 
     int submit(Request r, hipStream_t input_stream, long long *id) {
+        // the server's streams wait for the work already queued on the submitter's
+        // stream: an event recorded there before the request is queued (outside the
+        // lock: a runtime call the other threads need not wait for)
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            int rc = get_event(&r.ev_in);
+            if (rc) return rc;
+        }
+        if (hipEventRecord(r.ev_in, input_stream) != hipSuccess) {
+            std::lock_guard<std::mutex> lk(mu_);
+            put_event(r.ev_in);
+            return set_error(ESTL_ERR_HIP, "event record on the input stream failed");
+        }
         std::lock_guard<std::mutex> lk(mu_);
-        if (stop_) return set_error(ESTL_ERR_INVALID, "server is shutting down");
+        auto give_back = [&](int code) {
+            put_event(r.ev_in);
+            return code;
+        };
+        if (stop_) return give_back(set_error(ESTL_ERR_INVALID, "server is shutting down"));
         if ((long)queue_.size() >= max_queue_) {
             ++busy_;
-            return set_error(ESTL_ERR_BUSY, "request queue full (%ld)", max_queue_);
+            return give_back(set_error(ESTL_ERR_BUSY, "request queue full (%ld)", max_queue_));
         }
         Request *filter = nullptr;
         if (r.kind == KIND_CONV_REG) {
             auto it = reqs_.find(r.filter);
             if (it == reqs_.end() || it->second.kind != KIND_FILTER)
-                return set_error(ESTL_ERR_INVALID, "%lld is not a registered filter", r.filter);
-            if (it->second.state == FAILED) return set_error(ESTL_ERR_INVALID, "filter %lld failed", r.filter);
+                return give_back(set_error(ESTL_ERR_INVALID, "%lld is not a registered filter", r.filter));
+            if (it->second.state == FAILED) return give_back(set_error(ESTL_ERR_INVALID, "filter %lld failed", r.filter));
             filter = &it->second;
             r.n = filter->n;
             if ((filter->flags & ESTL_CONV_FP16) != (r.flags & ESTL_CONV_FP16))
-                return set_error(ESTL_ERR_INVALID, "filter %lld and the request differ in FP16 storage", r.filter);
-        }
-        // the server's streams wait for the work already queued on the submitter's stream
-        int rc = get_event(&r.ev_in);
-        if (rc) return rc;
-        if (hipEventRecord(r.ev_in, input_stream) != hipSuccess) {
-            put_event(r.ev_in);
-            return set_error(ESTL_ERR_HIP, "event record on the input stream failed");
+                return give_back(
+                    set_error(ESTL_ERR_INVALID, "filter %lld and the request differ in FP16 storage", r.filter));
         }
         if (filter) ++filter->users;
         r.id = ++next_id_;
@@ -321,18 +354,18 @@ class Server {
                  "\"queued\": %zu, \"running\": %zu, \"p50_ms\": %.4f, \"p99_ms\": %.4f, \"mean_ms\": %.4f, "
                  "\"max_ms\": %.4f, \"elapsed_s\": %.6f, \"throughput_req_s\": %.3f, \"throughput_gbs\": %.3f, "
                  "\"budget_bytes\": %zu, \"arena_used_bytes\": %zu, \"arena_peak_bytes\": %zu, \"peak_inflight\": %zu, "
-                 "\"streams\": %d}",
+                 "\"streams\": %d, \"groups\": %lld, \"grouped_requests\": %lld}",
                  submitted_, completed_, rejected_, failed_, busy_, queue_.size(), inflight_.size(), 1e3 * pct(50),
                  1e3 * pct(99), 1e3 * mean, lat.empty() ? -1.0 : 1e3 * lat.back(), elapsed,
                  elapsed > 0 ? completed_ / elapsed : 0.0, elapsed > 0 ? done_bytes_ / elapsed / 1e9 : 0.0,
-                 arena_.size(), arena_.used(), arena_.peak(), peak_inflight_, nstreams_);
+                 arena_.size(), arena_.used(), arena_.peak(), peak_inflight_, nstreams_, groups_, grouped_);
         return buf;
     }
 
     void reset_stats() {
         std::lock_guard<std::mutex> lk(mu_);
         latencies_.clear();
-        submitted_ = completed_ = rejected_ = failed_ = busy_ = 0;
+        submitted_ = completed_ = rejected_ = failed_ = busy_ = groups_ = grouped_ = 0;
         done_bytes_ = 0;
         first_submit_ = last_done_ = Clock::time_point();
         peak_inflight_ = inflight_.size();
@@ -385,6 +418,164 @@ class Server {
         return c.spec_plan.kind == Plan::PASSES ? c.spec_plan.scratch_bytes : 0;
     }
 
+    Plan *group_spec_plan(long n, long filters) {
+        auto key = std::make_pair(n, filters);
+        auto it = group_spec_plans_.find(key);
+        if (it != group_spec_plans_.end()) return it->second.get();
+        std::unique_ptr<Plan> p(new Plan());
+        if (group_spectrum_plan(p.get(), n, filters)) return nullptr;
+        Plan *raw = p.get();
+        group_spec_plans_[key] = std::move(p);
+        return raw;
+    }
+
+    // an FP32 multi-pass convolution whose filter is ready (inline, or registered and done)
+    bool groupable(const Request &r) const {
+        if (r.kind != KIND_CONV && r.kind != KIND_CONV_REG) return false;
+        if (r.flags & (ESTL_CONV_FP16 | ESTL_CONV_NORMALIZE_KERNEL)) return false;
+        if (r.n <= (1L << kMaxRowsLog2) || r.rows < 1) return false;
+        if (r.kind == KIND_CONV_REG) {
+            auto f = reqs_.find(r.filter);
+            if (f == reqs_.end() || f->second.state != DONE) return false;
+        }
+        return true;
+    }
+
+    // Admission of the queue head `r` as the leader of a group (under the lock): its
+    // companions from the next kGroupWindow queued requests, one output block each, and
+    // the group's workspace on the leader: two scratch buffers, the spectra (inline
+    // filters) and the device descriptor.  False (nothing changed) when there are no
+    // companions or the group does not fit now; r is then admitted alone.
+    bool admit_group(Request &r) {
+        if (free_slots_.empty() || !groupable(r)) return false;
+        std::vector<Request *> m{&r};
+        long pairs = (r.rows + 1) / 2, values = r.n * r.rows;
+        const size_t window = std::min(queue_.size(), (size_t)kGroupWindow);
+        for (size_t i = 1; i < window && (int)m.size() < kGroupMax; ++i) {
+            Request &o = reqs_.at(queue_[i]);
+            if (o.kind != r.kind || o.n != r.n || o.flags != r.flags || !groupable(o)) continue;
+            if (values + o.n * o.rows > kGroupMaxValues) continue;
+            m.push_back(&o);
+            pairs += (o.rows + 1) / 2;
+            values += o.n * o.rows;
+        }
+        if (m.size() < 2) return false;
+        const long G = (long)m.size();
+        const bool inline_filters = r.kind == KIND_CONV;
+        ConvShape *gs = conv_shape(r.n, 2 * pairs, r.flags);
+        Plan *gsp = inline_filters ? group_spec_plan(r.n, G) : nullptr;
+        if (!gs || gs->fused || gs->pairs != pairs || (inline_filters && !gsp)) return false;
+        // blocks: every member's output, then the leader's workspace
+        std::vector<size_t> sizes;
+        for (Request *q : m) sizes.push_back(sizeof(float) * (size_t)q->n * (size_t)q->rows);
+        const size_t scr = sizeof(float2) * (size_t)r.n * (size_t)std::max(pairs, G);
+        sizes.push_back(scr);
+        sizes.push_back(scr);
+        if (inline_filters) sizes.push_back(sizeof(float2) * (size_t)r.n * (size_t)G);
+        sizes.push_back(sizeof(ConvGroup));
+        size_t total = 0;
+        for (size_t b : sizes) total += (b + 255) & ~size_t(255);
+        if (total > arena_.size() - arena_.held()) return false;
+        std::vector<std::pair<size_t, size_t>> blocks;
+        for (size_t b : sizes) {
+            size_t off;
+            if (!arena_.alloc(b, &off)) {
+                for (auto &x : blocks) arena_.release(x.first, x.second);
+                return false;
+            }
+            blocks.emplace_back(off, b);
+        }
+        hipEvent_t ev = nullptr;
+        if (get_event(&ev)) {
+            for (auto &x : blocks) arena_.release(x.first, x.second);
+            return false;
+        }
+        // committed: out of the queue, outputs assigned, the leader carries the rest
+        for (size_t i = 0; i < m.size(); ++i) {
+            Request *q = m[i];
+            queue_.erase(std::find(queue_.begin(), queue_.end(), q->id));
+            q->blocks.assign(1, blocks[i]);
+            q->out = arena_.base() + blocks[i].first;
+            q->state = RUNNING;
+            q->alg_bytes = 8.0 * q->n * q->rows + (inline_filters ? 4.0 * q->k : 0.0);
+            if (q->kind == KIND_CONV_REG) q->spectrum = static_cast<const float2 *>(reqs_.at(q->filter).out);
+            if (i) {
+                q->leader = r.id;
+                r.followers.push_back(q->id);
+            }
+        }
+        for (size_t i = m.size(); i < blocks.size(); ++i) r.blocks.push_back(blocks[i]);
+        r.gshape = gs;
+        r.gspec = gsp;
+        r.gslot = free_slots_.back();
+        free_slots_.pop_back();
+        r.ev = ev;
+        r.stream = next_stream_++ % nstreams_;
+        ++groups_;
+        grouped_ += G;
+        ++launching_;
+        work_[r.stream].push_back(r.id);
+        cv_work_[r.stream].notify_one();
+        return true;
+    }
+
+    // The group's launches (launcher thread, no lock; `fol` resolved under the lock):
+    // wait for every follower's inputs, upload the descriptor from the group's pinned
+    // slot, the spectra of inline filters, then the convolution passes.
+    int launch_group(Request &r, const std::vector<Request *> &fol, hipStream_t st) {
+        for (Request *f : fol)
+            if (hipStreamWaitEvent(st, f->ev_in, 0) != hipSuccess)
+                return set_error(ESTL_ERR_HIP, "stream wait on a follower's input event failed");
+        const bool inline_filters = r.kind == KIND_CONV;
+        const long n = r.n;
+        char *base = static_cast<char *>(arena_.base());
+        // leader blocks: [0] output, [1] [2] scratch, [3] spectra (inline), last descriptor
+        float2 *scr0 = reinterpret_cast<float2 *>(base + r.blocks[1].first);
+        float2 *scr1 = reinterpret_cast<float2 *>(base + r.blocks[2].first);
+        float2 *spectra = inline_filters ? reinterpret_cast<float2 *>(base + r.blocks[3].first) : nullptr;
+        ConvGroup *d_desc = reinterpret_cast<ConvGroup *>(base + r.blocks.back().first);
+        ConvGroup &g = host_desc_[r.gslot];
+        memset(&g, 0, sizeof g);
+        g.count = 1 + (int)fol.size();
+        long pair = 0;
+        for (int q = 0; q < g.count; ++q) {
+            const Request &m = q ? *fol[q - 1] : r;
+            if (m.n != n || m.rows < 1) return set_error(ESTL_ERR_INVALID, "group member %d does not match", q);
+            g.pair_start[q] = pair;
+            pair += (m.rows + 1) / 2;
+            g.real_rows[q] = m.rows;
+            g.x[q] = static_cast<const float *>(m.in);
+            g.y[q] = static_cast<float *>(m.out);
+            g.spec[q] = inline_filters ? spectra + (size_t)q * n : m.spectrum;
+            g.h[q] = m.h;
+            g.k[q] = m.k;
+            g.shift[q] = (m.flags & ESTL_CONV_CENTER) ? ((m.k + 1) / 2) % n : 0;
+            if (!g.x[q] || !g.y[q] || !g.spec[q] || (inline_filters && (!g.h[q] || g.k[q] < 1)))
+                return set_error(ESTL_ERR_INVALID, "group member %d has a null buffer", q);
+        }
+        g.pair_start[g.count] = pair;
+        if (pair != r.gshape->pairs) return set_error(ESTL_ERR_INVALID, "group pairs %ld != shape pairs %ld", pair,
+                                                      r.gshape->pairs);
+        if (hipMemcpyAsync(d_desc, &g, sizeof g, hipMemcpyHostToDevice, st) != hipSuccess)
+            return set_error(ESTL_ERR_HIP, "group descriptor upload failed");
+        if (inline_filters) {
+            if (r.gspec->batch != g.count) return set_error(ESTL_ERR_INVALID, "group spectrum plan size");
+            if (int rc = conv_filter_group(*r.gspec, d_desc, spectra, scr0, st)) return rc;
+        }
+        return conv_run_group(*r.gshape, d_desc, scr0, scr1, st);
+    }
+
+    // a group leader and its followers fail together; the slot comes back
+    void fail_group_locked(Request &r, int code, const std::string &msg) {
+        for (long long id : r.followers) {
+            auto it = reqs_.find(id);
+            if (it != reqs_.end()) fail_locked(it->second, code, msg);
+        }
+        if (r.gslot >= 0) free_slots_.push_back(r.gslot);
+        r.gslot = -1;
+        fail_locked(r, code, msg);
+    }
+
     // sizes of the arena blocks a request needs: output first
     bool footprint(Request &r, std::vector<size_t> &sizes, std::string &why) {
         sizes.clear();
@@ -420,7 +611,6 @@ class Server {
                 }
                 if (r.kind == KIND_CONV) {
                     sizes.push_back(c->spec_bytes);
-                    sizes.push_back(c->spec_bytes);
                     sizes.push_back(256);
                 }
                 r.alg_bytes = ((r.flags & ESTL_CONV_FP16) ? 4.0 : 8.0) * r.n * r.rows + (r.kind == KIND_CONV ? 4.0 * r.k : 0.0);
@@ -430,7 +620,6 @@ class Server {
                 ConvShape *c = conv_shape(r.n, 1, r.flags);
                 if (!c) return fail();
                 r.shape = c;
-                sizes.push_back(c->spec_bytes);
                 sizes.push_back(c->spec_bytes);
                 sizes.push_back(256);
                 if (spec_scratch(*c)) sizes.push_back(spec_scratch(*c));
@@ -458,9 +647,8 @@ class Server {
             ConvShape *c = r.shape;
             ConvBuffers b;
             b.spec = ptr(0);
-            b.hp = ptr(1);
-            b.dc = ptr(2);
-            if (r.blocks.size() > 3) b.scratch[0] = ptr(3);
+            b.dc = ptr(1);
+            if (r.blocks.size() > 2) b.scratch[0] = ptr(2);
             return conv_filter(*c, b, r.h, r.k, st);
         }
         ConvShape *c = r.shape;
@@ -473,7 +661,6 @@ class Server {
         }
         if (r.kind == KIND_CONV) {
             b.spec = ptr(next++);
-            b.hp = ptr(next++);
             b.dc = ptr(next++);
             int rc = conv_filter(*c, b, r.h, r.k, st);
             if (rc) return rc;
@@ -529,6 +716,7 @@ class Server {
                     continue;
                 }
             }
+            if (admit_group(r)) continue;
             std::vector<size_t> sizes;
             std::string why;
             if (!footprint(r, sizes, why)) {
@@ -596,18 +784,23 @@ class Server {
             if (work_[s].empty()) return;  // stopping, nothing left to launch
             Request &r = reqs_.at(work_[s].front());
             work_[s].pop_front();
+            std::vector<Request *> fol;  // a group's followers (running: not erased meanwhile)
+            for (long long id : r.followers) fol.push_back(&reqs_.at(id));
             lk.unlock();
             int rc = ESTL_OK;
             if (hipStreamWaitEvent(st, r.ev_in, 0) != hipSuccess)
                 rc = set_error(ESTL_ERR_HIP, "stream wait on the input event failed");
-            if (!rc) rc = launch(r, st);
+            if (!rc) rc = fol.empty() ? launch(r, st) : launch_group(r, fol, st);
             if (!rc && hipEventRecord(r.ev, st) != hipSuccess) rc = set_error(ESTL_ERR_HIP, "event record failed");
             const std::string err = rc ? estl_last_error() : std::string();
             lk.lock();
             --launching_;
             if (rc) {
                 if (r.kind == KIND_FILTER) arena_.unhold(r.blocks[0].second);
-                fail_locked(r, rc, err);
+                if (fol.empty())
+                    fail_locked(r, rc, err);
+                else
+                    fail_group_locked(r, rc, err);
                 cv_dispatch_.notify_one();
                 continue;
             }
@@ -636,7 +829,11 @@ class Server {
                 put_event(r.ev_in);
                 if (q != hipSuccess) {
                     if (r.kind == KIND_FILTER) arena_.unhold(r.blocks[0].second);
-                    fail_locked(r, ESTL_ERR_HIP, std::string("device error: ") + hipGetErrorString(q));
+                    const std::string msg = std::string("device error: ") + hipGetErrorString(q);
+                    if (r.followers.empty())
+                        fail_locked(r, ESTL_ERR_HIP, msg);
+                    else
+                        fail_group_locked(r, ESTL_ERR_HIP, msg);
                 } else {
                     r.t_done = Clock::now();
                     r.state = DONE;
@@ -650,6 +847,20 @@ class Server {
                         last_done_ = r.t_done;
                     }
                     finish_locked(r);
+                    // a group completes with its leader
+                    for (long long id : r.followers) {
+                        Request &f = reqs_.at(id);
+                        f.t_done = r.t_done;
+                        f.state = DONE;
+                        put_event(f.ev_in);
+                        latencies_.push_back(std::chrono::duration<double>(f.t_done - f.t_submit).count());
+                        ++completed_;
+                        done_bytes_ += f.alg_bytes;
+                        finish_locked(f);
+                    }
+                    r.followers.clear();
+                    if (r.gslot >= 0) free_slots_.push_back(r.gslot);
+                    r.gslot = -1;
                 }
                 any = true;
             }
@@ -686,6 +897,11 @@ class Server {
     double done_bytes_ = 0;
     std::vector<double> latencies_;
     Clock::time_point first_submit_, last_done_;
+    // request groups
+    std::map<std::pair<long, long>, std::unique_ptr<Plan>> group_spec_plans_;  // (n, filters)
+    ConvGroup *host_desc_ = nullptr;  // kGroupSlots pinned descriptors
+    std::vector<int> free_slots_;
+    long long groups_ = 0, grouped_ = 0;
 };
 
 }  // namespace estl

@@ -84,6 +84,21 @@ def test_correlation_center_and_normalize(gpu, n):
     check_rows(y, oracle.conv_circular(x64, hp / hp.sum()), n)
 
 
+@pytest.mark.parametrize("n", [64, 2 ** 14])
+def test_long_filter_centred_and_normalised(gpu, n):
+    """A filter longer than n (folds modulo n) with the centre rotation and the kernel-sum
+    normalisation together: the spectrum kernel builds the folded, rotated filter as it
+    loads the taps (one workgroup for n <= 2^13, the first spectrum pass above)."""
+    import torch
+    rows, k = 3, n + 77
+    x = oracle.fill_uniform(rows * n, 12).astype(np.float32).reshape(rows, n)
+    h = (oracle.fill_uniform(k, 13).astype(np.float32) + 1.25).astype(np.float32)
+    hp = folded(h.astype(np.float64), n)
+    s = ((k + 1) // 2) % n
+    y = conv_gpu(torch, gpu, x, h, estl_amd.CONV_CENTER | estl_amd.CONV_NORMALIZE_KERNEL)
+    check_rows(y, oracle.conv_circular(x.astype(np.float64), np.roll(hp, -s) / hp.sum()), n)
+
+
 def test_identity_and_delay_closed_forms(gpu):
     import torch
     n, rows = 2 ** 14, 5

@@ -276,3 +276,55 @@ def test_fp16_requests_equal_fp16_object_bitwise(gpu):
         srv.release(rid)
     assert srv.stats()["failed"] == 0
     srv.close()
+
+
+@pytest.mark.parametrize("flags", [0, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER])
+def test_request_groups_equal_alone_bitwise(gpu, flags):
+    """Queued multi-pass convolutions of one length run as one group (one launch per
+    pass for all of them); every member's result is bit-identical to the request run
+    alone, with odd and even row counts, distinct inputs and filters (inline, then
+    registered), and requests of other lengths queued in between."""
+    import torch
+    n = 2 ** 15
+    rows_list = [1, 2, 3, 7, 16, 5, 1, 9]
+    ks = [16, 300, 4097, 64, 2048, 1, 999, 8192]
+    cases = [("conv", n, r, k, flags) for r, k in zip(rows_list, ks)]
+    other = ("conv", 2 ** 17, 3, 100, 0)
+    inputs = [make_inputs(torch, gpu, c, 300 + i) for i, c in enumerate(cases)]
+    xo, ho = make_inputs(torch, gpu, other, 399)
+    refs = [alone(torch, gpu, c, x, h) for c, (x, h) in zip(cases, inputs)]
+    ref_o = alone(torch, gpu, other, xo, ho)
+    srv = estl_amd.Server(4 << 30, streams=2)
+    ids = []
+    for i, c in enumerate(cases):
+        ids.append((i, submit(srv, c, *inputs[i])))
+        if i == 3:
+            ids.append((-1, submit(srv, other, xo, ho)))
+    srv.drain(120)
+    for i, rid in ids:
+        srv.wait(rid, 60)
+        y = result(srv, rid, other if i < 0 else cases[i], torch)
+        assert torch.equal(y, ref_o if i < 0 else refs[i]), (i, rid)
+        srv.release(rid)
+    st = srv.stats()
+    assert st["completed"] == len(ids) and st["failed"] == 0
+    assert st["groups"] >= 1 and st["grouped_requests"] >= 2, st
+    # registered filters: one per request, all ready before the requests queue up
+    # (centring belongs to the registration, correlation to each request)
+    fids = [srv.register_filter(n, h, int(h.numel()), flags & ~estl_amd.CONV_CORRELATE) for _, h in inputs]
+    for f in fids:
+        srv.wait(f, 60)
+    srv.reset_stats()
+    ids = [(i, srv.submit_conv_filter(fids[i], cases[i][2], inputs[i][0], flags & estl_amd.CONV_CORRELATE))
+           for i in range(len(cases))]
+    srv.drain(120)
+    for i, rid in ids:
+        srv.wait(rid, 60)
+        assert torch.equal(result(srv, rid, cases[i], torch), refs[i]), (i, rid)
+        srv.release(rid)
+    st = srv.stats()
+    assert st["failed"] == 0 and st["groups"] >= 1, st
+    for f in fids:
+        srv.release(f)
+    assert srv.stats()["arena_used_bytes"] == 0
+    srv.close()
