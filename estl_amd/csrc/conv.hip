@@ -38,18 +38,33 @@ hipError_t launch_spec_normalize(float2 *spec, long n, const float2 *dc, hipStre
     return hipGetLastError();
 }
 
+// The member of a request group that owns pair `pair` (pass_tile.h has the same rule).
+__device__ __forceinline__ int conv_group_member(const ConvGroup *g, long pair) {
+    int q = 0;
+    while (q + 1 < g->count && g->pair_start[q + 1] <= pair) ++q;
+    return q;
+}
+
 // The spectrum of a filter of length n <= 2^13 in one workgroup: fold the taps on
 // load, forward FFT (the rows kernel's block_fft and table, so the bins are those of the
 // rows plan), optionally divide by the kernel sum Re K^[0] (Convolution.h:141), store.
+// With a request group, workgroup q builds member q's filter into its spectrum.
 template <int LOG2N>
 __global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT)
 k_filter_spectrum(const float *__restrict__ h, long k, long shift, float2 *__restrict__ spec,
-                  const float2 *__restrict__ tw, int normalize) {
+                  const float2 *__restrict__ tw, int normalize, const ConvGroup *__restrict__ group) {
     using G = BlockGeom<LOG2N>;
     constexpr int N = G::N, E = G::E, TPT = G::TPT;
     __shared__ float2 lds[G::uses_lds ? Lds<kLdsPad>::size(N) : 1];
     __shared__ float s_dc;
     const int tl = opaque_lane(threadIdx.x);
+    if (group) {
+        const int q = blockIdx.x;
+        h = group->h[q];
+        k = group->k[q];
+        shift = group->shift[q];
+        spec = const_cast<float2 *>(group->spec[q]);
+    }
     float2 v[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) v[i] = make_float2(fold_tap(h, k, N, shift, tl + TPT * i), 0.0f);
@@ -66,12 +81,14 @@ k_filter_spectrum(const float *__restrict__ h, long k, long shift, float2 *__res
 }
 
 hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift, float2 *spec, const float2 *tw,
-                                  bool normalize, hipStream_t st) {
+                                  bool normalize, hipStream_t st, const ConvGroup *group, int members) {
+    if (group && (members < 1 || members > kGroupMax || normalize)) return hipErrorInvalidValue;
+    const unsigned blocks = group ? (unsigned)members : 1u;
     switch (log2n) {
-#define FS(L)                                                                                                  \
-    case L:                                                                                                    \
-        hipLaunchKernelGGL(k_filter_spectrum<L>, dim3(1), dim3(BlockGeom<L>::TPT), 0, st, h, k, shift, spec, tw, \
-                           normalize ? 1 : 0);                                                                 \
+#define FS(L)                                                                                                   \
+    case L:                                                                                                     \
+        hipLaunchKernelGGL(k_filter_spectrum<L>, dim3(blocks), dim3(BlockGeom<L>::TPT), 0, st, h, k, shift, spec, \
+                           tw, normalize ? 1 : 0, group);                                                        \
         break;
         FS(1) FS(2) FS(3) FS(4) FS(5) FS(6) FS(7) FS(8) FS(9) FS(10) FS(11) FS(12) FS(13)
 #undef FS
@@ -80,11 +97,13 @@ hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift,
     return hipGetLastError();
 }
 
-// Persistent over pairs of real rows (TPW pairs per workgroup), like k_fft_rows.
+// Persistent over pairs of real rows (TPW pairs per workgroup), like k_fft_rows.  With
+// a request group, `rows` counts the group's pairs twice and each slot takes its pair's
+// member: input rows, spectrum and output rows.
 template <int LOG2N, int TPW, bool CORR>
 __global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT * TPW)
 k_conv_rows(const float *__restrict__ x, float *__restrict__ y, long rows, const float2 *__restrict__ spec,
-            const float2 *__restrict__ tw, float scale) {
+            const float2 *__restrict__ tw, float scale, const ConvGroup *__restrict__ group) {
     using G = BlockGeom<LOG2N>;
     constexpr int N = G::N, E = G::E, TPT = G::TPT;
     constexpr int SLICE = G::uses_lds ? Lds<kLdsPad>::size(N) : 0;
@@ -97,8 +116,16 @@ k_conv_rows(const float *__restrict__ x, float *__restrict__ y, long rows, const
         const int w = t / TPT, tl = t % TPT;
         const long pr = grp * TPW + w;
         const bool active = pr < pairs;
-        const long r1 = active ? 2 * pr : 0;
-        const bool two = active && r1 + 1 < rows;
+        long r1 = active ? 2 * pr : 0, nrows = rows;
+        if (group) {
+            const int q = active ? conv_group_member(group, pr) : 0;
+            r1 = active ? 2 * (pr - group->pair_start[q]) : 0;
+            nrows = group->real_rows[q];
+            x = group->x[q];
+            y = group->y[q];
+            spec = group->spec[q];
+        }
+        const bool two = active && r1 + 1 < nrows;
         const float *x1 = x + r1 * N + tl;
         float2 v[E];
 #pragma unroll
@@ -227,7 +254,7 @@ static long conv_resident(K kernel, int threads) {
 
 template <int LOG2N, bool CORR>
 static hipError_t launch_conv_rows_t(const float *x, float *y, long rows, const float2 *spec, const float2 *tw,
-                                     float scale, hipStream_t st) {
+                                     float scale, hipStream_t st, const ConvGroup *group) {
     using G = BlockGeom<LOG2N>;
     constexpr int TPW = (G::TPT >= 256) ? 1 : 256 / G::TPT;
     auto kernel = k_conv_rows<LOG2N, TPW, CORR>;
@@ -236,7 +263,7 @@ static hipError_t launch_conv_rows_t(const float *x, float *y, long rows, const 
     const long pairs = (rows + 1) / 2;
     long blocks = (pairs + TPW - 1) / TPW;
     if (blocks > 4 * resident) blocks = 4 * resident;
-    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(G::TPT * TPW), 0, st, x, y, rows, spec, tw, scale);
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(G::TPT * TPW), 0, st, x, y, rows, spec, tw, scale, group);
     return hipGetLastError();
 }
 
@@ -277,10 +304,10 @@ hipError_t launch_conv_rows_fp16(int log2n, bool corr, const void *x, void *y, l
 
 template <bool CORR>
 static hipError_t launch_conv_rows_c(int log2n, const float *x, float *y, long rows, const float2 *spec,
-                                     const float2 *tw, float scale, hipStream_t st) {
+                                     const float2 *tw, float scale, hipStream_t st, const ConvGroup *group) {
     switch (log2n) {
 #define CONV_CASE(L) \
-    case L: return launch_conv_rows_t<L, CORR>(x, y, rows, spec, tw, scale, st);
+    case L: return launch_conv_rows_t<L, CORR>(x, y, rows, spec, tw, scale, st, group);
         CONV_CASE(1) CONV_CASE(2) CONV_CASE(3) CONV_CASE(4) CONV_CASE(5) CONV_CASE(6) CONV_CASE(7) CONV_CASE(8)
         CONV_CASE(9) CONV_CASE(10) CONV_CASE(11) CONV_CASE(12) CONV_CASE(13)
 #undef CONV_CASE
@@ -289,9 +316,9 @@ static hipError_t launch_conv_rows_c(int log2n, const float *x, float *y, long r
 }
 
 hipError_t launch_conv_rows(int log2n, bool corr, const float *x, float *y, long rows, const float2 *spec,
-                            const float2 *tw, float scale, hipStream_t st) {
-    return corr ? launch_conv_rows_c<true>(log2n, x, y, rows, spec, tw, scale, st)
-                : launch_conv_rows_c<false>(log2n, x, y, rows, spec, tw, scale, st);
+                            const float2 *tw, float scale, hipStream_t st, const ConvGroup *group) {
+    return corr ? launch_conv_rows_c<true>(log2n, x, y, rows, spec, tw, scale, st, group)
+                : launch_conv_rows_c<false>(log2n, x, y, rows, spec, tw, scale, st, group);
 }
 
 }  // namespace estl

@@ -23,10 +23,11 @@
 
 namespace estl {
 
+// (group: a request group's descriptor in device memory, or null)
 hipError_t launch_conv_rows(int log2n, bool corr, const float *x, float *y, long rows, const float2 *spec,
-                            const float2 *tw, float scale, hipStream_t st);
+                            const float2 *tw, float scale, hipStream_t st, const ConvGroup *group = nullptr);
 hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift, float2 *spec, const float2 *tw,
-                                  bool normalize, hipStream_t st);
+                                  bool normalize, hipStream_t st, const ConvGroup *group = nullptr, int members = 0);
 hipError_t launch_spec_normalize(float2 *spec, long n, const float2 *dc, hipStream_t st);
 hipError_t launch_conv_rows_fp16(int log2n, bool corr, const void *x, void *y, long rows, const float2 *spec,
                                  const float2 *tw, float scale, hipStream_t st);
@@ -201,6 +202,14 @@ int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y,
 // A request group (ConvGroup in device memory at d_group).  Spectra: the G filters as
 // the G rows of `spec` (a spectrum plan over G rows whose first pass builds them from
 // their taps), destinations alternating back from `spectra` with scratch0.
+int conv_filter_group_rows(const ConvShape &c, const ConvGroup *d_group, int members, hipStream_t st) {
+    if (!c.fused || c.spec_plan.kind != Plan::ROWS) return set_error(ESTL_ERR_INVALID, "not a fused shape");
+    hipError_t e = launch_filter_spectrum(c.log2n, nullptr, 0, 0, nullptr, c.spec_plan.tw_rows, false, st, d_group,
+                                          members);
+    if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group filter spectra: %s", hipGetErrorString(e));
+    return ESTL_OK;
+}
+
 int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectra, float2 *scratch0, hipStream_t st) {
     if (spec.kind != Plan::PASSES || spec.steps.empty() || spec.steps.front().args.io_in != 3)
         return set_error(ESTL_ERR_INVALID, "not a group spectrum plan");
@@ -221,8 +230,13 @@ int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectr
 // back to back): the first pass reads each member's rows, the middle pass multiplies by
 // each member's spectrum, the last writes each member's rows.
 int conv_run_group(const ConvShape &c, const ConvGroup *d_group, float2 *scratch0, float2 *scratch1, hipStream_t st) {
-    if (c.fused || c.precision != ESTL_PRECISION_FP32)
-        return set_error(ESTL_ERR_INVALID, "request groups run FP32 multi-pass convolutions");
+    if (c.precision != ESTL_PRECISION_FP32) return set_error(ESTL_ERR_INVALID, "request groups run FP32 convolutions");
+    if (c.fused) {  // one launch over the group's pairs (c.rows counts them twice)
+        hipError_t e = launch_conv_rows(c.log2n, c.corr, nullptr, nullptr, c.rows, nullptr, c.tw_rows, c.scale, st,
+                                        d_group);
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group k_conv_rows launch: %s", hipGetErrorString(e));
+        return ESTL_OK;
+    }
     float2 *scr[2] = {scratch0, scratch1};
     int cur = 0;
     const float2 *src = scratch1;  // not read by the first pass (it reads the members' rows)

@@ -12,8 +12,8 @@
 //   * continuous batching: admitted requests are launched at once on a pool of streams
 //     and run concurrently with the ones already on the device; their memory is
 //     returned the moment they complete, admitting the next;
-//   * request groups: a multi-pass FP32 convolution at the head of the queue takes
-//     along the queued requests of the same kind, length and flags found in the next
+//   * request groups: an FP32 convolution at the head of the queue takes along the
+//     queued requests of the same kind, length and flags found in the next
 //     kGroupWindow entries (up to kGroupMax requests, kGroupMaxValues values), and the
 //     group runs as one batch: one launch per pass for all of them (the pass kernels
 //     find each row's request in a ConvGroup descriptor) instead of one chain of
@@ -429,11 +429,11 @@ class Server {
         return raw;
     }
 
-    // an FP32 multi-pass convolution whose filter is ready (inline, or registered and done)
+    // an FP32 convolution whose filter is ready (inline, or registered and done)
     bool groupable(const Request &r) const {
         if (r.kind != KIND_CONV && r.kind != KIND_CONV_REG) return false;
         if (r.flags & (ESTL_CONV_FP16 | ESTL_CONV_NORMALIZE_KERNEL)) return false;
-        if (r.n <= (1L << kMaxRowsLog2) || r.rows < 1) return false;
+        if (r.n < 2 || r.rows < 1) return false;
         if (r.kind == KIND_CONV_REG) {
             auto f = reqs_.find(r.filter);
             if (f == reqs_.end() || f->second.state != DONE) return false;
@@ -463,14 +463,19 @@ class Server {
         const long G = (long)m.size();
         const bool inline_filters = r.kind == KIND_CONV;
         ConvShape *gs = conv_shape(r.n, 2 * pairs, r.flags);
-        Plan *gsp = inline_filters ? group_spec_plan(r.n, G) : nullptr;
-        if (!gs || gs->fused || gs->pairs != pairs || (inline_filters && !gsp)) return false;
-        // blocks: every member's output, then the leader's workspace
+        if (!gs || gs->pairs != pairs) return false;
+        // multi-pass sizes: the spectra through a spectrum plan over the G filters
+        Plan *gsp = (inline_filters && !gs->fused) ? group_spec_plan(r.n, G) : nullptr;
+        if (inline_filters && !gs->fused && !gsp) return false;
+        // blocks: every member's output, then the leader's workspace (multi-pass: two
+        // scratch buffers; inline filters: the spectra; the descriptor)
         std::vector<size_t> sizes;
         for (Request *q : m) sizes.push_back(sizeof(float) * (size_t)q->n * (size_t)q->rows);
-        const size_t scr = sizeof(float2) * (size_t)r.n * (size_t)std::max(pairs, G);
-        sizes.push_back(scr);
-        sizes.push_back(scr);
+        if (!gs->fused) {
+            const size_t scr = sizeof(float2) * (size_t)r.n * (size_t)std::max(pairs, G);
+            sizes.push_back(scr);
+            sizes.push_back(scr);
+        }
         if (inline_filters) sizes.push_back(sizeof(float2) * (size_t)r.n * (size_t)G);
         sizes.push_back(sizeof(ConvGroup));
         size_t total = 0;
@@ -527,12 +532,19 @@ class Server {
             if (hipStreamWaitEvent(st, f->ev_in, 0) != hipSuccess)
                 return set_error(ESTL_ERR_HIP, "stream wait on a follower's input event failed");
         const bool inline_filters = r.kind == KIND_CONV;
+        const bool fused = r.gshape->fused;
         const long n = r.n;
         char *base = static_cast<char *>(arena_.base());
-        // leader blocks: [0] output, [1] [2] scratch, [3] spectra (inline), last descriptor
-        float2 *scr0 = reinterpret_cast<float2 *>(base + r.blocks[1].first);
-        float2 *scr1 = reinterpret_cast<float2 *>(base + r.blocks[2].first);
-        float2 *spectra = inline_filters ? reinterpret_cast<float2 *>(base + r.blocks[3].first) : nullptr;
+        // leader blocks: [0] output, then (multi-pass) two scratch buffers, (inline
+        // filters) the spectra, and last the descriptor
+        size_t next = 1;
+        float2 *scr0 = nullptr, *scr1 = nullptr, *spectra = nullptr;
+        if (!fused) {
+            scr0 = reinterpret_cast<float2 *>(base + r.blocks[next++].first);
+            scr1 = reinterpret_cast<float2 *>(base + r.blocks[next++].first);
+        }
+        if (inline_filters) spectra = reinterpret_cast<float2 *>(base + r.blocks[next++].first);
+        if (next + 1 != r.blocks.size()) return set_error(ESTL_ERR_INVALID, "group workspace blocks");
         ConvGroup *d_desc = reinterpret_cast<ConvGroup *>(base + r.blocks.back().first);
         ConvGroup &g = host_desc_[r.gslot];
         memset(&g, 0, sizeof g);
@@ -558,7 +570,9 @@ class Server {
                                                       r.gshape->pairs);
         if (hipMemcpyAsync(d_desc, &g, sizeof g, hipMemcpyHostToDevice, st) != hipSuccess)
             return set_error(ESTL_ERR_HIP, "group descriptor upload failed");
-        if (inline_filters) {
+        if (inline_filters && fused) {
+            if (int rc = conv_filter_group_rows(*r.gshape, d_desc, g.count, st)) return rc;
+        } else if (inline_filters) {
             if (r.gspec->batch != g.count) return set_error(ESTL_ERR_INVALID, "group spectrum plan size");
             if (int rc = conv_filter_group(*r.gspec, d_desc, spectra, scr0, st)) return rc;
         }

@@ -278,27 +278,42 @@ def test_fp16_requests_equal_fp16_object_bitwise(gpu):
     srv.close()
 
 
-@pytest.mark.parametrize("flags", [0, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER])
-def test_request_groups_equal_alone_bitwise(gpu, flags):
-    """Queued multi-pass convolutions of one length run as one group (one launch per
-    pass for all of them); every member's result is bit-identical to the request run
-    alone, with odd and even row counts, distinct inputs and filters (inline, then
-    registered), and requests of other lengths queued in between."""
+def _hold(torch):
+    """A stream whose queue is busy for a while (a bounded device sleep): requests
+    submitted with it as their input stream stay in flight until it ends, so the ones
+    submitted after them queue up behind the server's in-flight limit."""
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        torch.cuda._sleep(20_000_000)
+    return side
+
+
+@pytest.mark.parametrize("n,flags", [(2 ** 15, 0), (2 ** 15, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER),
+                                     (2 ** 12, 0), (2 ** 13, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER)])
+def test_request_groups_equal_alone_bitwise(gpu, n, flags):
+    """Queued convolutions of one length run as one group (one launch per pass, or one
+    fused launch, for all of them); every member's result is bit-identical to the
+    request run alone, with odd and even row counts, distinct inputs and filters
+    (inline, then registered), and a request of another length queued in between.  One
+    stream (two requests in flight) whose first two requests wait on a busy input
+    stream, so the rest queue up and group."""
     import torch
-    n = 2 ** 15
-    rows_list = [1, 2, 3, 7, 16, 5, 1, 9]
-    ks = [16, 300, 4097, 64, 2048, 1, 999, 8192]
+    rows_list = [1, 2, 3, 7, 16, 5, 1, 9, 4, 2]
+    ks = [16, 300, 4097, 64, 2048, 1, 999, 8192, 33, 100]
     cases = [("conv", n, r, k, flags) for r, k in zip(rows_list, ks)]
     other = ("conv", 2 ** 17, 3, 100, 0)
     inputs = [make_inputs(torch, gpu, c, 300 + i) for i, c in enumerate(cases)]
     xo, ho = make_inputs(torch, gpu, other, 399)
     refs = [alone(torch, gpu, c, x, h) for c, (x, h) in zip(cases, inputs)]
     ref_o = alone(torch, gpu, other, xo, ho)
-    srv = estl_amd.Server(4 << 30, streams=2)
+    srv = estl_amd.Server(4 << 30, streams=1)
+    side = _hold(torch)
     ids = []
     for i, c in enumerate(cases):
-        ids.append((i, submit(srv, c, *inputs[i])))
-        if i == 3:
+        kind, n_, rows, k, fl = c
+        x, h = inputs[i]
+        ids.append((i, srv.submit_conv(n_, rows, x, h, k, fl, stream=side if i < 2 else None)))
+        if i == 5:
             ids.append((-1, submit(srv, other, xo, ho)))
     srv.drain(120)
     for i, rid in ids:
@@ -308,22 +323,23 @@ def test_request_groups_equal_alone_bitwise(gpu, flags):
         srv.release(rid)
     st = srv.stats()
     assert st["completed"] == len(ids) and st["failed"] == 0
-    assert st["groups"] >= 1 and st["grouped_requests"] >= 2, st
+    assert st["groups"] >= 1 and st["grouped_requests"] >= 4, st
     # registered filters: one per request, all ready before the requests queue up
     # (centring belongs to the registration, correlation to each request)
     fids = [srv.register_filter(n, h, int(h.numel()), flags & ~estl_amd.CONV_CORRELATE) for _, h in inputs]
     for f in fids:
         srv.wait(f, 60)
     srv.reset_stats()
-    ids = [(i, srv.submit_conv_filter(fids[i], cases[i][2], inputs[i][0], flags & estl_amd.CONV_CORRELATE))
-           for i in range(len(cases))]
+    side = _hold(torch)
+    ids = [(i, srv.submit_conv_filter(fids[i], cases[i][2], inputs[i][0], flags & estl_amd.CONV_CORRELATE,
+                                      stream=side if i < 2 else None)) for i in range(len(cases))]
     srv.drain(120)
     for i, rid in ids:
         srv.wait(rid, 60)
         assert torch.equal(result(srv, rid, cases[i], torch), refs[i]), (i, rid)
         srv.release(rid)
     st = srv.stats()
-    assert st["failed"] == 0 and st["groups"] >= 1, st
+    assert st["failed"] == 0 and st["groups"] >= 1 and st["grouped_requests"] >= 4, st
     for f in fids:
         srv.release(f)
     assert srv.stats()["arena_used_bytes"] == 0
