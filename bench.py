@@ -383,8 +383,10 @@ def bench_serve(torch, estl_amd, stream, load: float = 0.7, budget: int = 16 << 
             "budget_bytes": budget, "streams": streams,
             "capacity_req_s": round(capacity, 1), "capacity_gbs": round(alg_bytes / wall_cap / 1e9, 1),
             "capacity_p50_ms": cap["p50_ms"], "capacity_p99_ms": cap["p99_ms"],
+            "capacity_groups": cap["groups"], "capacity_grouped_requests": cap["grouped_requests"],
             "offered_req_s": round(rate, 1), "load": load,
             "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "load_grouped_requests": st["grouped_requests"],
             "throughput_req_s": round(count / wall, 1), "throughput_gbs": round(alg_bytes / wall / 1e9, 1),
             "plain_per_request_req_s": round(count / wall_plain, 1),
             "served_over_plain_per_request": round(wall_plain / wall_cap, 2),
