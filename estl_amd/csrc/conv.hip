@@ -121,8 +121,8 @@ k_conv_rows(const float *__restrict__ x, float *__restrict__ y, long rows, const
             const int q = active ? conv_group_member(group, pr) : 0;
             r1 = active ? 2 * (pr - group->pair_start[q]) : 0;
             nrows = group->real_rows[q];
-            x = group->x[q];
-            y = group->y[q];
+            x = static_cast<const float *>(group->x[q]);
+            y = static_cast<float *>(group->y[q]);
             spec = group->spec[q];
         }
         const bool two = active && r1 + 1 < nrows;
@@ -155,7 +155,7 @@ k_conv_rows(const float *__restrict__ x, float *__restrict__ y, long rows, const
 template <int LOG2N, int TPW, bool CORR>
 __global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT * TPW)
 k_conv_rows_h(const _Float16 *__restrict__ x, _Float16 *__restrict__ y, long rows, const float2 *__restrict__ spec,
-              const float2 *__restrict__ tw, float scale) {
+              const float2 *__restrict__ tw, float scale, const ConvGroup *__restrict__ group) {
     using G = BlockGeom<LOG2N>;
     constexpr int N = G::N, E = G::E, TPT = G::TPT;
     constexpr int SLICE = G::uses_lds ? Lds<kLdsPad>::size(N) : 0;
@@ -168,8 +168,16 @@ k_conv_rows_h(const _Float16 *__restrict__ x, _Float16 *__restrict__ y, long row
         const int w = t / TPT, tl = t % TPT;
         const long pr = grp * TPW + w;
         const bool active = pr < pairs;
-        const long r1 = active ? 2 * pr : 0;
-        const bool two = active && r1 + 1 < rows;
+        long r1 = active ? 2 * pr : 0, nrows = rows;
+        if (group) {  // the pair's member, as in k_conv_rows
+            const int q = active ? conv_group_member(group, pr) : 0;
+            r1 = active ? 2 * (pr - group->pair_start[q]) : 0;
+            nrows = group->real_rows[q];
+            x = static_cast<const _Float16 *>(group->x[q]);
+            y = static_cast<_Float16 *>(group->y[q]);
+            spec = group->spec[q];
+        }
+        const bool two = active && r1 + 1 < nrows;
         const _Float16 *x1 = x + r1 * N + tl;
         float2 v[E];
 #pragma unroll
@@ -242,6 +250,58 @@ hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1,
     return hipGetLastError();
 }
 
+// The same three steps for every member of a request group (workgroup row y = member),
+// in the same order and arithmetic as for one request, the (sum |h|, L) pair of member q
+// kept in g->l1[q].
+__global__ void __launch_bounds__(256) k_filter_l1_group(ConvGroup *__restrict__ g) {
+    const int q = blockIdx.x;
+    const float *h = g->h[q];
+    const long k = g->k[q];
+    __shared__ float part[256];
+    float acc = 0.0f;
+    for (long i = threadIdx.x; i < k; i += 256) acc += fabsf(h[i]);
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) g->l1[q].y = part[0];
+}
+
+__global__ void k_spec_l1_group(ConvGroup *__restrict__ g, long n) {
+    const int q = blockIdx.y;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float L = g->l1[q].y;
+    if (!(L > 0.0f)) L = 1.0f;
+    if (i < n) {
+        float2 *spec = const_cast<float2 *>(g->spec[q]);
+        const float s = 1.0f / L;
+        spec[i] = make_float2(spec[i].x * s, spec[i].y * s);
+    }
+    if (i == 0) g->l1[q].x = L;
+}
+
+__global__ void k_conv_scales_group(const ConvGroup *__restrict__ g) {
+    const int q = blockIdx.y;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < g->real_rows[q]) g->yscale[q][i] = g->xscale[q][i] * g->l1[q].x;
+}
+
+hipError_t launch_conv_fp16_filter_group(ConvGroup *g, int members, long n, hipStream_t st) {
+    if (members < 1 || members > kGroupMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_filter_l1_group, dim3((unsigned)members), dim3(256), 0, st, g);
+    hipLaunchKernelGGL(k_spec_l1_group, dim3((unsigned)((n + 255) / 256), (unsigned)members), dim3(256), 0, st, g, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv_scales_group(const ConvGroup *g, int members, long max_rows, hipStream_t st) {
+    if (members < 1 || members > kGroupMax || max_rows < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_conv_scales_group, dim3((unsigned)((max_rows + 255) / 256), (unsigned)members), dim3(256), 0,
+                       st, g);
+    return hipGetLastError();
+}
+
 template <class K>
 static long conv_resident(K kernel, int threads) {
     int per_cu = 0, cus = 0, dev = 0;
@@ -269,7 +329,7 @@ static hipError_t launch_conv_rows_t(const float *x, float *y, long rows, const 
 
 template <int LOG2N, bool CORR>
 static hipError_t launch_conv_rows_h_t(const void *x, void *y, long rows, const float2 *spec, const float2 *tw,
-                                       float scale, hipStream_t st) {
+                                       float scale, hipStream_t st, const ConvGroup *group) {
     using G = BlockGeom<LOG2N>;
     constexpr int TPW = (G::TPT >= 256) ? 1 : 256 / G::TPT;
     auto kernel = k_conv_rows_h<LOG2N, TPW, CORR>;
@@ -279,16 +339,16 @@ static hipError_t launch_conv_rows_h_t(const void *x, void *y, long rows, const 
     long blocks = (pairs + TPW - 1) / TPW;
     if (blocks > 4 * resident) blocks = 4 * resident;
     hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(G::TPT * TPW), 0, st, static_cast<const _Float16 *>(x),
-                       static_cast<_Float16 *>(y), rows, spec, tw, scale);
+                       static_cast<_Float16 *>(y), rows, spec, tw, scale, group);
     return hipGetLastError();
 }
 
 template <bool CORR>
 static hipError_t launch_conv_rows_h_c(int log2n, const void *x, void *y, long rows, const float2 *spec,
-                                       const float2 *tw, float scale, hipStream_t st) {
+                                       const float2 *tw, float scale, hipStream_t st, const ConvGroup *group) {
     switch (log2n) {
 #define CONV_H_CASE(L) \
-    case L: return launch_conv_rows_h_t<L, CORR>(x, y, rows, spec, tw, scale, st);
+    case L: return launch_conv_rows_h_t<L, CORR>(x, y, rows, spec, tw, scale, st, group);
         CONV_H_CASE(1) CONV_H_CASE(2) CONV_H_CASE(3) CONV_H_CASE(4) CONV_H_CASE(5) CONV_H_CASE(6) CONV_H_CASE(7)
         CONV_H_CASE(8) CONV_H_CASE(9) CONV_H_CASE(10) CONV_H_CASE(11) CONV_H_CASE(12) CONV_H_CASE(13)
 #undef CONV_H_CASE
@@ -297,9 +357,9 @@ static hipError_t launch_conv_rows_h_c(int log2n, const void *x, void *y, long r
 }
 
 hipError_t launch_conv_rows_fp16(int log2n, bool corr, const void *x, void *y, long rows, const float2 *spec,
-                                 const float2 *tw, float scale, hipStream_t st) {
-    return corr ? launch_conv_rows_h_c<true>(log2n, x, y, rows, spec, tw, scale, st)
-                : launch_conv_rows_h_c<false>(log2n, x, y, rows, spec, tw, scale, st);
+                                 const float2 *tw, float scale, hipStream_t st, const ConvGroup *group) {
+    return corr ? launch_conv_rows_h_c<true>(log2n, x, y, rows, spec, tw, scale, st, group)
+                : launch_conv_rows_h_c<false>(log2n, x, y, rows, spec, tw, scale, st, group);
 }
 
 template <bool CORR>

@@ -30,7 +30,9 @@ hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift,
                                   bool normalize, hipStream_t st, const ConvGroup *group = nullptr, int members = 0);
 hipError_t launch_spec_normalize(float2 *spec, long n, const float2 *dc, hipStream_t st);
 hipError_t launch_conv_rows_fp16(int log2n, bool corr, const void *x, void *y, long rows, const float2 *spec,
-                                 const float2 *tw, float scale, hipStream_t st);
+                                 const float2 *tw, float scale, hipStream_t st, const ConvGroup *group = nullptr);
+hipError_t launch_conv_fp16_filter_group(ConvGroup *g, int members, long n, hipStream_t st);
+hipError_t launch_conv_scales_group(const ConvGroup *g, int members, long max_rows, hipStream_t st);
 hipError_t launch_conv_fp16_filter(const float *h, long k, float2 *spec, long n, const float2 *dc, bool normalized,
                                    hipStream_t st);
 hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1, long rows, hipStream_t st);
@@ -262,6 +264,47 @@ int conv_run_group(const ConvShape &c, const ConvGroup *d_group, float2 *scratch
     if (int rc = run(c.mid, 0)) return rc;
     for (const auto &s : c.inv)
         if (int rc = run(s, ESTL_BACKWARD)) return rc;
+    return ESTL_OK;
+}
+
+// FP16 storage: every member's spectrum divided by its filter's L = sum |h| (kept in
+// the descriptor), the passes on binary16 rows, then the members' output row scales.
+int conv_fp16_filter_group(ConvGroup *d_group, int members, long n, hipStream_t st) {
+    hipError_t e = launch_conv_fp16_filter_group(d_group, members, n, st);
+    if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group FP16 filter scaling: %s", hipGetErrorString(e));
+    return ESTL_OK;
+}
+
+int conv_run_group_fp16(const ConvShape &c, ConvGroup *d_group, int members, long max_rows, void *scratch0,
+                        void *scratch1, hipStream_t st) {
+    if (c.precision != ESTL_PRECISION_FP16) return set_error(ESTL_ERR_INVALID, "not an FP16-storage shape");
+    hipError_t e;
+    if (c.fused) {
+        e = launch_conv_rows_fp16(c.log2n, c.corr, nullptr, nullptr, c.rows, nullptr, c.tw_rows, c.scale, st, d_group);
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group k_conv_rows_h launch: %s", hipGetErrorString(e));
+    } else {
+        void *scr[2] = {scratch0, scratch1};
+        int cur = 0;
+        const void *src = scratch1;  // not read by the first pass (it reads the members' rows)
+        auto run = [&](const PassStep &s, int dir) -> int {
+            PassArgs a = s.args;
+            a.group = d_group;
+            void *dst = scr[cur];
+            hipError_t le = dir ? launch_pass_fp16(s.log2b, s.log2t, dir, src, dst, a, st)
+                                : launch_pass_mid_fp16(s.log2b, s.log2t, src, dst, a, st);
+            if (le != hipSuccess) return set_error(ESTL_ERR_HIP, "group FP16 pass launch: %s", hipGetErrorString(le));
+            src = dst;
+            cur ^= 1;
+            return ESTL_OK;
+        };
+        for (const auto &s : c.fwd)
+            if (int rc = run(s, ESTL_FORWARD)) return rc;
+        if (int rc = run(c.mid, 0)) return rc;
+        for (const auto &s : c.inv)
+            if (int rc = run(s, ESTL_BACKWARD)) return rc;
+    }
+    e = launch_conv_scales_group(d_group, members, max_rows, st);
+    if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group output scales: %s", hipGetErrorString(e));
     return ESTL_OK;
 }
 

@@ -10,20 +10,26 @@ namespace estl {
 // A group of convolution requests of one length n run as one batch (the server's
 // continuous batching): request q owns pair rows [pair_start[q], pair_start[q + 1]) of
 // the batch.  Its real rows are read from x[q] and written to y[q] (real_rows[q] of
-// them), its spectrum is spec[q]; in the group's spectrum passes row q is the filter of
-// request q (taps h[q], k[q] of them, rotation shift[q]).  Lives in device memory.
+// them; FP32, or binary16 with FP16 storage), its spectrum is spec[q] (n bins); in the
+// group's spectrum passes row q is the filter of request q (taps h[q], k[q] of them,
+// rotation shift[q]).  FP16 storage: xscale[q] / yscale[q] are the member's input and
+// output row scales and l1[q] its filter's (sum |h|, L) pair, filled on the device.
+// Lives in device memory.
 constexpr int kGroupMax = 16;
 struct ConvGroup {
     int count;
     int pad_;
     long pair_start[kGroupMax + 1];
     long real_rows[kGroupMax];
-    const float *x[kGroupMax];
-    float *y[kGroupMax];
+    const void *x[kGroupMax];
+    void *y[kGroupMax];
     const float2 *spec[kGroupMax];
     const float *h[kGroupMax];
     long k[kGroupMax];
     long shift[kGroupMax];
+    const float *xscale[kGroupMax];
+    float *yscale[kGroupMax];
+    float2 l1[kGroupMax];
 };
 
 struct PassArgs {
@@ -55,7 +61,7 @@ struct PassArgs {
     int planar_out;
     const float2 *spectrum;
     long fk, fshift;         // io_in = 3: taps and rotation of the filter
-    // FP32 convolution passes of a request group: rows index the group's batch, and the
+    // convolution passes of a request group: rows index the group's batch, and the
     // real-pair I/O, the filter loads and the middle pass's spectrum are the member's
     const ConvGroup *group;
 };

@@ -80,8 +80,8 @@ __device__ __forceinline__ void pass_tile_load(const ST *__restrict__ in, const 
         // FP32 or, with FP16 storage, binary16 reals; only the first pass reads pairs
         // and its plans keep S >= T.  In a request group: the member's pair and rows.
         long prow = P.row, rrows = a.real_rows;
-        const float *xb = a.rin;
-        if (F32 && a.group) {
+        const void *xb = F32 ? static_cast<const void *>(a.rin) : a.rin16;
+        if (a.group) {
             const int q = group_member(a.group, P.row);
             prow = P.row - a.group->pair_start[q];
             rrows = a.group->real_rows[q];
@@ -90,14 +90,14 @@ __device__ __forceinline__ void pass_tile_load(const ST *__restrict__ in, const 
         const bool two = 2 * prow + 1 < rrows;
         const int base = (P.g << (ls + LOG2B)) + P.z;
         if constexpr (F32) {
-            const float *x1 = xb + ((2 * prow) << a.log2n);
+            const float *x1 = static_cast<const float *>(xb) + ((2 * prow) << a.log2n);
 #pragma unroll
             for (int i = 0; i < E; ++i) {
                 const int o = base + ((P.tl + TPT * i) << ls);
                 v[i] = make_float2(ld_stream_f(x1 + o), two ? ld_stream_f(x1 + (1L << a.log2n) + o) : 0.0f);
             }
         } else {
-            const _Float16 *x1 = static_cast<const _Float16 *>(a.rin16) + ((2 * P.row) << a.log2n);
+            const _Float16 *x1 = static_cast<const _Float16 *>(xb) + ((2 * prow) << a.log2n);
 #pragma unroll
             for (int i = 0; i < E; ++i) {
                 const int o = base + ((P.tl + TPT * i) << ls);
@@ -212,8 +212,8 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
         // re / im to real rows 2 row and 2 row + 1 of rout (FP32 or binary16); in a
         // request group, the member's pair and rows
         long prow = P.row, rrows = a.real_rows;
-        float *yb = a.rout;
-        if (F32 && a.group) {
+        void *yb = F32 ? static_cast<void *>(a.rout) : a.rout16;
+        if (a.group) {
             const int q = group_member(a.group, P.row);
             prow = P.row - a.group->pair_start[q];
             rrows = a.group->real_rows[q];
@@ -221,7 +221,7 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
         }
         const bool two = 2 * prow + 1 < rrows;
         if constexpr (F32) {
-            float *y1 = yb + ((2 * prow) << a.log2n) + j;
+            float *y1 = static_cast<float *>(yb) + ((2 * prow) << a.log2n) + j;
 #pragma unroll
             for (int i = 0; i < E; ++i) {
                 const int o = (tl + TPT * i) << lcpr;
@@ -229,7 +229,7 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
                 if (two) st_stream_f(y1 + (1L << a.log2n) + o, v[i].y * a.scale);
             }
         } else {
-            _Float16 *y1 = static_cast<_Float16 *>(a.rout16) + ((2 * P.row) << a.log2n) + j;
+            _Float16 *y1 = static_cast<_Float16 *>(yb) + ((2 * prow) << a.log2n) + j;
 #pragma unroll
             for (int i = 0; i < E; ++i) {
                 const int o = (tl + TPT * i) << lcpr;
@@ -241,7 +241,7 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
     }
     if constexpr (MID) {
         // the last forward pass yields natural order: output h of column j is bin j + h n/B
-        const float2 *spec = (F32 && a.group) ? a.group->spec[group_member(a.group, P.row)] : a.spectrum;
+        const float2 *spec = a.group ? a.group->spec[group_member(a.group, P.row)] : a.spectrum;
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const float2 hk = spec[j + ((tl + TPT * i) << lcpr)];

@@ -429,10 +429,12 @@ class Server {
         return raw;
     }
 
-    // an FP32 convolution whose filter is ready (inline, or registered and done)
+    // a convolution whose filter is ready (inline, or registered and done; FP16 storage
+    // with inline filters only: a group keeps their L1 norms in its descriptor)
     bool groupable(const Request &r) const {
         if (r.kind != KIND_CONV && r.kind != KIND_CONV_REG) return false;
-        if (r.flags & (ESTL_CONV_FP16 | ESTL_CONV_NORMALIZE_KERNEL)) return false;
+        if (r.flags & ESTL_CONV_NORMALIZE_KERNEL) return false;
+        if ((r.flags & ESTL_CONV_FP16) && r.kind != KIND_CONV) return false;
         if (r.n < 2 || r.rows < 1) return false;
         if (r.kind == KIND_CONV_REG) {
             auto f = reqs_.find(r.filter);
@@ -469,10 +471,14 @@ class Server {
         if (inline_filters && !gs->fused && !gsp) return false;
         // blocks: every member's output, then the leader's workspace (multi-pass: two
         // scratch buffers; inline filters: the spectra; the descriptor)
+        const bool fp16 = (r.flags & ESTL_CONV_FP16) != 0;
         std::vector<size_t> sizes;
-        for (Request *q : m) sizes.push_back(sizeof(float) * (size_t)q->n * (size_t)q->rows);
-        if (!gs->fused) {
-            const size_t scr = sizeof(float2) * (size_t)r.n * (size_t)std::max(pairs, G);
+        for (Request *q : m)  // outputs as footprint() lays them out
+            sizes.push_back(fp16 ? fp16_out_scales_offset(q->n, q->rows) + sizeof(float) * (size_t)q->rows
+                                 : sizeof(float) * (size_t)q->n * (size_t)q->rows);
+        if (!gs->fused) {  // the passes' ping-pong, also the FP32 spectrum plan's
+            const size_t scr = std::max(gs->scratch_bytes, inline_filters ? sizeof(float2) * (size_t)r.n * (size_t)G
+                                                                          : size_t(0));
             sizes.push_back(scr);
             sizes.push_back(scr);
         }
@@ -502,7 +508,7 @@ class Server {
             q->blocks.assign(1, blocks[i]);
             q->out = arena_.base() + blocks[i].first;
             q->state = RUNNING;
-            q->alg_bytes = 8.0 * q->n * q->rows + (inline_filters ? 4.0 * q->k : 0.0);
+            q->alg_bytes = (fp16 ? 4.0 : 8.0) * q->n * q->rows + (inline_filters ? 4.0 * q->k : 0.0);
             if (q->kind == KIND_CONV_REG) q->spectrum = static_cast<const float2 *>(reqs_.at(q->filter).out);
             if (i) {
                 q->leader = r.id;
@@ -533,6 +539,7 @@ class Server {
                 return set_error(ESTL_ERR_HIP, "stream wait on a follower's input event failed");
         const bool inline_filters = r.kind == KIND_CONV;
         const bool fused = r.gshape->fused;
+        const bool fp16 = (r.flags & ESTL_CONV_FP16) != 0;
         const long n = r.n;
         char *base = static_cast<char *>(arena_.base());
         // leader blocks: [0] output, then (multi-pass) two scratch buffers, (inline
@@ -549,20 +556,26 @@ class Server {
         ConvGroup &g = host_desc_[r.gslot];
         memset(&g, 0, sizeof g);
         g.count = 1 + (int)fol.size();
-        long pair = 0;
+        long pair = 0, max_rows = 0;
         for (int q = 0; q < g.count; ++q) {
             const Request &m = q ? *fol[q - 1] : r;
             if (m.n != n || m.rows < 1) return set_error(ESTL_ERR_INVALID, "group member %d does not match", q);
             g.pair_start[q] = pair;
             pair += (m.rows + 1) / 2;
+            max_rows = std::max(max_rows, m.rows);
             g.real_rows[q] = m.rows;
-            g.x[q] = static_cast<const float *>(m.in);
-            g.y[q] = static_cast<float *>(m.out);
+            g.x[q] = m.in;
+            g.y[q] = m.out;
             g.spec[q] = inline_filters ? spectra + (size_t)q * n : m.spectrum;
             g.h[q] = m.h;
             g.k[q] = m.k;
             g.shift[q] = (m.flags & ESTL_CONV_CENTER) ? ((m.k + 1) / 2) % n : 0;
-            if (!g.x[q] || !g.y[q] || !g.spec[q] || (inline_filters && (!g.h[q] || g.k[q] < 1)))
+            if (fp16) {
+                g.xscale[q] = m.xscale;
+                g.yscale[q] = reinterpret_cast<float *>(static_cast<char *>(m.out) + fp16_out_scales_offset(n, m.rows));
+            }
+            if (!g.x[q] || !g.y[q] || !g.spec[q] || (inline_filters && (!g.h[q] || g.k[q] < 1)) ||
+                (fp16 && !g.xscale[q]))
                 return set_error(ESTL_ERR_INVALID, "group member %d has a null buffer", q);
         }
         g.pair_start[g.count] = pair;
@@ -575,6 +588,10 @@ class Server {
         } else if (inline_filters) {
             if (r.gspec->batch != g.count) return set_error(ESTL_ERR_INVALID, "group spectrum plan size");
             if (int rc = conv_filter_group(*r.gspec, d_desc, spectra, scr0, st)) return rc;
+        }
+        if (fp16) {
+            if (int rc = conv_fp16_filter_group(d_desc, g.count, n, st)) return rc;
+            return conv_run_group_fp16(*r.gshape, d_desc, g.count, max_rows, scr0, scr1, st);
         }
         return conv_run_group(*r.gshape, d_desc, scr0, scr1, st);
     }

@@ -344,3 +344,45 @@ def test_request_groups_equal_alone_bitwise(gpu, n, flags):
         srv.release(f)
     assert srv.stats()["arena_used_bytes"] == 0
     srv.close()
+
+
+@pytest.mark.parametrize("n,flags", [(2 ** 16, estl_amd.CONV_CORRELATE), (2 ** 11, estl_amd.CONV_CENTER)])
+def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags):
+    """FP16-storage requests of one length, queued behind a busy input stream, run as a
+    group (the filters' L1 norms kept in the group's descriptor); each binary16 output
+    and its row scales are bit-identical to the FP16 convolution object's."""
+    import torch
+    rows_list = [3, 1, 8, 5, 2, 7]
+    ks = [100, 16, 2000, 33, 8192, 1]
+    refs, keep = [], []
+    for i, (rows, k) in enumerate(zip(rows_list, ks)):
+        x = torch.empty(rows * n, dtype=torch.float32, device=gpu)
+        estl_amd.fill_uniform(x, 500 + i)
+        x16 = torch.empty(rows * n, dtype=torch.float16, device=gpu)
+        s_in = torch.empty(rows, dtype=torch.float32, device=gpu)
+        estl_amd.pack_fp16(x, x16, s_in, n // 2, rows)
+        h = torch.empty(k, dtype=torch.float32, device=gpu)
+        estl_amd.fill_uniform(h, 600 + i)
+        c = estl_amd.Convolution(n, rows, flags, precision=estl_amd.FP16)
+        c.set_filter(h)
+        y16, s_out = torch.empty_like(x16), torch.empty_like(s_in)
+        c.execute_fp16(x16, s_in, y16, s_out)
+        torch.cuda.synchronize()
+        refs.append((y16.clone(), s_out.clone()))
+        keep.append((x16, s_in, h))
+    srv = estl_amd.Server(1 << 30, streams=1)
+    side = _hold(torch)
+    ids = [srv.submit_conv_fp16(n, rows, x16, s_in, h, int(h.numel()), flags, stream=side if i < 2 else None)
+           for i, (rows, (x16, s_in, h)) in enumerate(zip(rows_list, keep))]
+    srv.drain(120)
+    for i, rid in enumerate(ids):
+        srv.wait(rid, 60)
+        y16, s_out = srv.output_fp16(rid, n, rows_list[i])
+        torch.cuda.synchronize()
+        assert torch.equal(y16, refs[i][0]), i
+        assert torch.equal(s_out, refs[i][1]), (i, s_out.tolist(), refs[i][1].tolist())
+        srv.release(rid)
+    st = srv.stats()
+    assert st["failed"] == 0 and st["groups"] >= 1 and st["grouped_requests"] >= 3, st
+    assert st["arena_used_bytes"] == 0
+    srv.close()
