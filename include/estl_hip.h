@@ -322,8 +322,8 @@ int estl_server_release(estl_server_t srv, long long request_id);
 /* Wait until nothing is queued or running. */
 int estl_server_drain(estl_server_t srv, double timeout_s);
 
-/* JSON statistics (counts, p50/p99/mean/max latency in ms, throughput, arena use);
- * returns the full length. */
+/* JSON statistics (counts, p50/p99/mean/max latency in ms, throughput, arena use,
+ * request groups formed and the requests they served); returns the full length. */
 int estl_server_stats(estl_server_t srv, char *json, int len);
 int estl_server_reset_stats(estl_server_t srv);
 
