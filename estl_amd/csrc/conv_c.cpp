@@ -233,13 +233,17 @@ int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectr
     return ESTL_OK;
 }
 
-// The convolution passes of a request group over the shape's pairs (the members' pairs
+// The convolution passes of a request group over its `pairs` pairs (the members' pairs
 // back to back): the first pass reads each member's rows, the middle pass multiplies by
-// each member's spectrum, the last writes each member's rows.
-int conv_run_group(const ConvShape &c, const ConvGroup *d_group, float2 *scratch0, float2 *scratch1, hipStream_t st) {
+// each member's spectrum, the last writes each member's rows.  The shape may have been
+// built for another batch: a pass's arithmetic and tiles do not depend on it, only the
+// grid, which follows `pairs`.
+int conv_run_group(const ConvShape &c, const ConvGroup *d_group, long pairs, float2 *scratch0, float2 *scratch1,
+                   hipStream_t st) {
     if (c.precision != ESTL_PRECISION_FP32) return set_error(ESTL_ERR_INVALID, "request groups run FP32 convolutions");
-    if (c.fused) {  // one launch over the group's pairs (c.rows counts them twice)
-        hipError_t e = launch_conv_rows(c.log2n, c.corr, nullptr, nullptr, c.rows, nullptr, c.tw_rows, c.scale, st,
+    if (pairs < 1) return set_error(ESTL_ERR_INVALID, "empty request group");
+    if (c.fused) {  // one launch over the group's pairs
+        hipError_t e = launch_conv_rows(c.log2n, c.corr, nullptr, nullptr, 2 * pairs, nullptr, c.tw_rows, c.scale, st,
                                         d_group);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group k_conv_rows launch: %s", hipGetErrorString(e));
         return ESTL_OK;
@@ -250,6 +254,7 @@ int conv_run_group(const ConvShape &c, const ConvGroup *d_group, float2 *scratch
     auto run = [&](const PassStep &s, int dir) -> int {
         PassArgs a = s.args;
         a.group = d_group;
+        a.batch = pairs;
         if (a.io_out == 2) a.scale = c.scale;
         float2 *dst = scr[cur];
         hipError_t e = dir ? launch_pass(s.log2b, s.log2t, dir, src, dst, a, st)
@@ -275,12 +280,14 @@ int conv_fp16_filter_group(ConvGroup *d_group, int members, long n, hipStream_t 
     return ESTL_OK;
 }
 
-int conv_run_group_fp16(const ConvShape &c, ConvGroup *d_group, int members, long max_rows, void *scratch0,
-                        void *scratch1, hipStream_t st) {
+int conv_run_group_fp16(const ConvShape &c, ConvGroup *d_group, int members, long pairs, long max_rows,
+                        void *scratch0, void *scratch1, hipStream_t st) {
     if (c.precision != ESTL_PRECISION_FP16) return set_error(ESTL_ERR_INVALID, "not an FP16-storage shape");
+    if (pairs < 1) return set_error(ESTL_ERR_INVALID, "empty request group");
     hipError_t e;
     if (c.fused) {
-        e = launch_conv_rows_fp16(c.log2n, c.corr, nullptr, nullptr, c.rows, nullptr, c.tw_rows, c.scale, st, d_group);
+        e = launch_conv_rows_fp16(c.log2n, c.corr, nullptr, nullptr, 2 * pairs, nullptr, c.tw_rows, c.scale, st,
+                                  d_group);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group k_conv_rows_h launch: %s", hipGetErrorString(e));
     } else {
         void *scr[2] = {scratch0, scratch1};
@@ -289,6 +296,7 @@ int conv_run_group_fp16(const ConvShape &c, ConvGroup *d_group, int members, lon
         auto run = [&](const PassStep &s, int dir) -> int {
             PassArgs a = s.args;
             a.group = d_group;
+            a.batch = pairs;
             void *dst = scr[cur];
             hipError_t le = dir ? launch_pass_fp16(s.log2b, s.log2t, dir, src, dst, a, st)
                                 : launch_pass_mid_fp16(s.log2b, s.log2t, src, dst, a, st);

@@ -119,9 +119,10 @@ int group_spectrum_plan(Plan *p, long n, long filters);
 int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectra, float2 *scratch0, hipStream_t st);
 int conv_filter_group_rows(const ConvShape &c, const ConvGroup *d_group, int members, hipStream_t st);
 int conv_fp16_filter_group(ConvGroup *d_group, int members, long n, hipStream_t st);
-int conv_run_group_fp16(const ConvShape &c, ConvGroup *d_group, int members, long max_rows, void *scratch0,
-                        void *scratch1, hipStream_t st);
-int conv_run_group(const ConvShape &c, const ConvGroup *d_group, float2 *scratch0, float2 *scratch1, hipStream_t st);
+int conv_run_group_fp16(const ConvShape &c, ConvGroup *d_group, int members, long pairs, long max_rows,
+                        void *scratch0, void *scratch1, hipStream_t st);
+int conv_run_group(const ConvShape &c, const ConvGroup *d_group, long pairs, float2 *scratch0, float2 *scratch1,
+                   hipStream_t st);
 int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y, hipStream_t st);
 // FP16 storage: x16 / y16 binary16 rows, y = (stored y) * s_out[row]
 int conv_run_fp16(const ConvShape &c, const ConvBuffers &b, const void *x16, const float *s_in, void *y16,

@@ -146,7 +146,8 @@ struct Request {
     // request group: the leader launches and completes it; followers ride along
     std::vector<long long> followers;  // leader
     long long leader = 0;              // follower
-    ConvShape *gshape = nullptr;       // leader: the shape over the group's pairs
+    ConvShape *gshape = nullptr;       // leader: the group shape of (n, flags, pair bucket)
+    long gpairs = 0;                   // leader: the group's pairs
     Plan *gspec = nullptr;             // leader: spectrum plan over the group's filters
     int gslot = -1;                    // leader: its pinned descriptor slot
 };
@@ -464,8 +465,13 @@ class Server {
         if (m.size() < 2) return false;
         const long G = (long)m.size();
         const bool inline_filters = r.kind == KIND_CONV;
-        ConvShape *gs = conv_shape(r.n, 2 * pairs, r.flags);
-        if (!gs || gs->pairs != pairs) return false;
+        // shapes for groups are cached per (n, flags, power of two of the pair count):
+        // built for 2^b >= pairs pairs (tiles fit a batch within 2x of the group's) and
+        // launched with the group's own pair count, so the cache stays bounded
+        int bucket = 0;
+        while ((1L << bucket) < pairs) ++bucket;
+        ConvShape *gs = conv_shape(r.n, 2 * (1L << bucket), r.flags);
+        if (!gs) return false;
         // multi-pass sizes: the spectra through a spectrum plan over the G filters
         Plan *gsp = (inline_filters && !gs->fused) ? group_spec_plan(r.n, G) : nullptr;
         if (inline_filters && !gs->fused && !gsp) return false;
@@ -476,9 +482,9 @@ class Server {
         for (Request *q : m)  // outputs as footprint() lays them out
             sizes.push_back(fp16 ? fp16_out_scales_offset(q->n, q->rows) + sizeof(float) * (size_t)q->rows
                                  : sizeof(float) * (size_t)q->n * (size_t)q->rows);
-        if (!gs->fused) {  // the passes' ping-pong, also the FP32 spectrum plan's
-            const size_t scr = std::max(gs->scratch_bytes, inline_filters ? sizeof(float2) * (size_t)r.n * (size_t)G
-                                                                          : size_t(0));
+        if (!gs->fused) {  // the passes' ping-pong (FP32 or binary16 pairs), also the FP32 spectrum plan's
+            const size_t scr = std::max((fp16 ? 4 : 8) * (size_t)r.n * (size_t)pairs,
+                                        inline_filters ? sizeof(float2) * (size_t)r.n * (size_t)G : size_t(0));
             sizes.push_back(scr);
             sizes.push_back(scr);
         }
@@ -518,6 +524,7 @@ class Server {
         for (size_t i = m.size(); i < blocks.size(); ++i) r.blocks.push_back(blocks[i]);
         r.gshape = gs;
         r.gspec = gsp;
+        r.gpairs = pairs;
         r.gslot = free_slots_.back();
         free_slots_.pop_back();
         r.ev = ev;
@@ -579,8 +586,7 @@ class Server {
                 return set_error(ESTL_ERR_INVALID, "group member %d has a null buffer", q);
         }
         g.pair_start[g.count] = pair;
-        if (pair != r.gshape->pairs) return set_error(ESTL_ERR_INVALID, "group pairs %ld != shape pairs %ld", pair,
-                                                      r.gshape->pairs);
+        if (pair != r.gpairs) return set_error(ESTL_ERR_INVALID, "group pairs %ld != admitted pairs %ld", pair, r.gpairs);
         if (hipMemcpyAsync(d_desc, &g, sizeof g, hipMemcpyHostToDevice, st) != hipSuccess)
             return set_error(ESTL_ERR_HIP, "group descriptor upload failed");
         if (inline_filters && fused) {
@@ -591,9 +597,9 @@ class Server {
         }
         if (fp16) {
             if (int rc = conv_fp16_filter_group(d_desc, g.count, n, st)) return rc;
-            return conv_run_group_fp16(*r.gshape, d_desc, g.count, max_rows, scr0, scr1, st);
+            return conv_run_group_fp16(*r.gshape, d_desc, g.count, pair, max_rows, scr0, scr1, st);
         }
-        return conv_run_group(*r.gshape, d_desc, scr0, scr1, st);
+        return conv_run_group(*r.gshape, d_desc, pair, scr0, scr1, st);
     }
 
     // a group leader and its followers fail together; the slot comes back
