@@ -415,7 +415,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
-              f"(the workload shards by batch row with no exchange; see the sharding note of the design)",
+              f"(the batched workload would shard by batch row, each GPU transforming its own rows "
+              f"with no exchange between GPUs, so its throughput is N times the one-GPU value)",
               file=sys.stderr)
         return 2
 
