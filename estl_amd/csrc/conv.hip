@@ -304,12 +304,7 @@ hipError_t launch_conv_scales_group(const ConvGroup *g, int members, long max_ro
 
 template <class K>
 static long conv_resident(K kernel, int threads) {
-    int per_cu = 0, cus = 0, dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
-        per_cu = 1;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
-    return (long)per_cu * cus;
+    return co_resident_blocks(reinterpret_cast<const void *>(kernel), threads);
 }
 
 template <int LOG2N, bool CORR>
@@ -318,8 +313,7 @@ static hipError_t launch_conv_rows_t(const float *x, float *y, long rows, const 
     using G = BlockGeom<LOG2N>;
     constexpr int TPW = (G::TPT >= 256) ? 1 : 256 / G::TPT;
     auto kernel = k_conv_rows<LOG2N, TPW, CORR>;
-    static long resident = 0;  // per process; one device per process on this path
-    if (!resident) resident = conv_resident(kernel, G::TPT * TPW);
+    const long resident = conv_resident(kernel, G::TPT * TPW);
     const long pairs = (rows + 1) / 2;
     long blocks = (pairs + TPW - 1) / TPW;
     if (blocks > 4 * resident) blocks = 4 * resident;
@@ -333,8 +327,7 @@ static hipError_t launch_conv_rows_h_t(const void *x, void *y, long rows, const 
     using G = BlockGeom<LOG2N>;
     constexpr int TPW = (G::TPT >= 256) ? 1 : 256 / G::TPT;
     auto kernel = k_conv_rows_h<LOG2N, TPW, CORR>;
-    static long resident = 0;  // per process; one device per process on this path
-    if (!resident) resident = conv_resident(kernel, G::TPT * TPW);
+    const long resident = conv_resident(kernel, G::TPT * TPW);
     const long pairs = (rows + 1) / 2;
     long blocks = (pairs + TPW - 1) / TPW;
     if (blocks > 4 * resident) blocks = 4 * resident;

@@ -171,18 +171,9 @@ static hipError_t launch_rows_h_t(const f16x2 *in, f16x2 *out, long batch, const
     using G = BlockGeom<LOG2N>;
     constexpr int TPW = (G::TPT >= 256) ? 1 : 256 / G::TPT;
     auto kernel = k_fft_rows_h<LOG2N, DIR, TPW>;
-    static long resident[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if (!resident[dev]) {
-        int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, G::TPT * TPW, 0) != hipSuccess || per_cu < 1)
-            per_cu = 1;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
-        resident[dev] = (long)per_cu * cus;
-    }
+    const long resident = co_resident_blocks(reinterpret_cast<const void *>(kernel), G::TPT * TPW);
     const long groups = (batch + TPW - 1) / TPW;
-    long blocks = 4 * resident[dev];
+    long blocks = 4 * resident;
     if (blocks > groups) blocks = groups;
     hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(G::TPT * TPW), 0, st, in, out, batch, tw, scale);
     return hipGetLastError();

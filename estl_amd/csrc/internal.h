@@ -68,6 +68,9 @@ struct PassArgs {
 
 hipError_t launch_rows(int log2n, int dir, const float2 *in, float2 *out, long batch, const float2 *tw, float scale,
                        hipStream_t st);
+// workgroups of a kernel co-resident on the current device (cached per kernel and
+// device, thread-safe; kernels.hip)
+long co_resident_blocks(const void *kernel, int block_threads);
 // rows kernel on the tile-planarized layout (kernels.hip)
 hipError_t launch_rows_planar(int log2n, int dir, const float2 *in, float2 *out, long batch, const float2 *tw,
                               float scale, hipStream_t st);

@@ -158,14 +158,13 @@ k_fft_rows_planar(const float2 *__restrict__ in, float2 *__restrict__ out, long 
 // ------------------------------------------------------------------ launchers
 // Workgroups of `kernel` resident on the whole device at once (occupancy x CUs),
 // cached per kernel (by address: kernels of one signature share a pointer type) and
-// device.
-template <class K>
-static long resident_blocks(K kernel, int block_threads) {
+// device, under a lock (launches come from several threads and devices).
+long co_resident_blocks(const void *kernel, int block_threads) {
     static std::mutex mu;
     static std::map<std::pair<const void *, int>, long> cache;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
-    const auto key = std::make_pair(reinterpret_cast<const void *>(kernel), dev);
+    const auto key = std::make_pair(kernel, dev);
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
@@ -174,6 +173,11 @@ static long resident_blocks(K kernel, int block_threads) {
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
     return cache[key] = (long)per_cu * cus;
+}
+
+template <class K>
+static long resident_blocks(K kernel, int block_threads) {
+    return co_resident_blocks(reinterpret_cast<const void *>(kernel), block_threads);
 }
 
 // persistent grid: this many resident rounds of workgroups
