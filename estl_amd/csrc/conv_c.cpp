@@ -171,29 +171,44 @@ int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k
     return ESTL_OK;
 }
 
-int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y, hipStream_t st) {
-    if (!x || !y) return set_error(ESTL_ERR_INVALID, "null buffer");
-    if (c.precision != ESTL_PRECISION_FP32)
-        return set_error(ESTL_ERR_INVALID, "an FP16-storage convolution runs through estl_conv_execute_fp16");
-    if (c.fused) {
-        hipError_t e = launch_conv_rows(c.log2n, c.corr, x, y, c.rows, b.spec, c.tw_rows, c.scale, st);
-        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "k_conv_rows launch: %s", hipGetErrorString(e));
-        return ESTL_OK;
-    }
-    // forward passes: x (real pairs) -> s0 -> s1 -> ..., the last multiplies by the
-    // spectrum; the inverse passes continue the ping-pong, the last writes y (real pairs)
+// The multi-pass chain of shape c: forward passes (the first reads the real pairs) ->
+// middle pass (last forward pass, spectrum multiply, first inverse pass) -> inverse
+// passes (the last writes the real pairs), ping-ponging between two scratch buffers.
+// One request: rows x -> y with spectrum `spec`; a request group: every member's rows
+// and spectrum from `group`, over `pairs` pairs (0: the shape's batch).  FP32 or
+// binary16 storage by the shape's precision (FP32's last pass takes the 1/n scale here;
+// binary16 passes carry their own 1/sqrt(radix) scaling).
+static int conv_pass_chain(const ConvShape &c, const void *x, void *y, const float2 *spec, const ConvGroup *group,
+                           long pairs, void *scratch0, void *scratch1, hipStream_t st) {
+    const bool h = c.precision == ESTL_PRECISION_FP16;
+    void *scr[2] = {scratch0, scratch1};
     int cur = 0;
-    const float2 *src = b.scratch[1];  // not read by the first pass (it reads x)
+    const void *src = scratch1;  // not read by the first pass (it reads the real pairs)
     auto run = [&](const PassStep &s, int dir) -> int {
         PassArgs a = s.args;
-        a.rin = x;
-        a.rout = y;
-        a.spectrum = b.spec;
-        if (a.io_out == 2) a.scale = c.scale;
-        float2 *dst = b.scratch[cur];
-        hipError_t e = dir ? launch_pass(s.log2b, s.log2t, dir, src, dst, a, st)
-                           : launch_pass_mid(s.log2b, s.log2t, src, dst, a, st);
-        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "conv pass launch: %s", hipGetErrorString(e));
+        if (h) {
+            a.rin16 = x;
+            a.rout16 = y;
+        } else {
+            a.rin = static_cast<const float *>(x);
+            a.rout = static_cast<float *>(y);
+            if (a.io_out == 2) a.scale = c.scale;
+        }
+        a.spectrum = spec;
+        a.group = group;
+        if (pairs > 0) a.batch = pairs;
+        void *dst = scr[cur];
+        hipError_t e;
+        if (h)
+            e = dir ? launch_pass_fp16(s.log2b, s.log2t, dir, src, dst, a, st)
+                    : launch_pass_mid_fp16(s.log2b, s.log2t, src, dst, a, st);
+        else
+            e = dir ? launch_pass(s.log2b, s.log2t, dir, static_cast<const float2 *>(src), static_cast<float2 *>(dst), a,
+                                  st)
+                    : launch_pass_mid(s.log2b, s.log2t, static_cast<const float2 *>(src), static_cast<float2 *>(dst), a,
+                                      st);
+        if (e != hipSuccess)
+            return set_error(ESTL_ERR_HIP, "%sconvolution pass launch: %s", h ? "FP16 " : "", hipGetErrorString(e));
         src = dst;
         cur ^= 1;
         return ESTL_OK;
@@ -204,6 +219,18 @@ int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y,
     for (const auto &s : c.inv)
         if (int rc = run(s, ESTL_BACKWARD)) return rc;
     return ESTL_OK;
+}
+
+int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y, hipStream_t st) {
+    if (!x || !y) return set_error(ESTL_ERR_INVALID, "null buffer");
+    if (c.precision != ESTL_PRECISION_FP32)
+        return set_error(ESTL_ERR_INVALID, "an FP16-storage convolution runs through estl_conv_execute_fp16");
+    if (c.fused) {
+        hipError_t e = launch_conv_rows(c.log2n, c.corr, x, y, c.rows, b.spec, c.tw_rows, c.scale, st);
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "k_conv_rows launch: %s", hipGetErrorString(e));
+        return ESTL_OK;
+    }
+    return conv_pass_chain(c, x, y, b.spec, nullptr, 0, b.scratch[0], b.scratch[1], st);
 }
 
 // A request group (ConvGroup in device memory at d_group).  Spectra: the G filters as
@@ -248,28 +275,7 @@ int conv_run_group(const ConvShape &c, const ConvGroup *d_group, long pairs, flo
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group k_conv_rows launch: %s", hipGetErrorString(e));
         return ESTL_OK;
     }
-    float2 *scr[2] = {scratch0, scratch1};
-    int cur = 0;
-    const float2 *src = scratch1;  // not read by the first pass (it reads the members' rows)
-    auto run = [&](const PassStep &s, int dir) -> int {
-        PassArgs a = s.args;
-        a.group = d_group;
-        a.batch = pairs;
-        if (a.io_out == 2) a.scale = c.scale;
-        float2 *dst = scr[cur];
-        hipError_t e = dir ? launch_pass(s.log2b, s.log2t, dir, src, dst, a, st)
-                           : launch_pass_mid(s.log2b, s.log2t, src, dst, a, st);
-        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group conv pass launch: %s", hipGetErrorString(e));
-        src = dst;
-        cur ^= 1;
-        return ESTL_OK;
-    };
-    for (const auto &s : c.fwd)
-        if (int rc = run(s, ESTL_FORWARD)) return rc;
-    if (int rc = run(c.mid, 0)) return rc;
-    for (const auto &s : c.inv)
-        if (int rc = run(s, ESTL_BACKWARD)) return rc;
-    return ESTL_OK;
+    return conv_pass_chain(c, nullptr, nullptr, nullptr, d_group, pairs, scratch0, scratch1, st);
 }
 
 // FP16 storage: every member's spectrum divided by its filter's L = sum |h| (kept in
@@ -289,27 +295,8 @@ int conv_run_group_fp16(const ConvShape &c, ConvGroup *d_group, int members, lon
         e = launch_conv_rows_fp16(c.log2n, c.corr, nullptr, nullptr, 2 * pairs, nullptr, c.tw_rows, c.scale, st,
                                   d_group);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group k_conv_rows_h launch: %s", hipGetErrorString(e));
-    } else {
-        void *scr[2] = {scratch0, scratch1};
-        int cur = 0;
-        const void *src = scratch1;  // not read by the first pass (it reads the members' rows)
-        auto run = [&](const PassStep &s, int dir) -> int {
-            PassArgs a = s.args;
-            a.group = d_group;
-            a.batch = pairs;
-            void *dst = scr[cur];
-            hipError_t le = dir ? launch_pass_fp16(s.log2b, s.log2t, dir, src, dst, a, st)
-                                : launch_pass_mid_fp16(s.log2b, s.log2t, src, dst, a, st);
-            if (le != hipSuccess) return set_error(ESTL_ERR_HIP, "group FP16 pass launch: %s", hipGetErrorString(le));
-            src = dst;
-            cur ^= 1;
-            return ESTL_OK;
-        };
-        for (const auto &s : c.fwd)
-            if (int rc = run(s, ESTL_FORWARD)) return rc;
-        if (int rc = run(c.mid, 0)) return rc;
-        for (const auto &s : c.inv)
-            if (int rc = run(s, ESTL_BACKWARD)) return rc;
+    } else if (int rc = conv_pass_chain(c, nullptr, nullptr, nullptr, d_group, pairs, scratch0, scratch1, st)) {
+        return rc;
     }
     e = launch_conv_scales_group(d_group, members, max_rows, st);
     if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group output scales: %s", hipGetErrorString(e));
@@ -324,27 +311,8 @@ int conv_run_fp16(const ConvShape &c, const ConvBuffers &b, const void *x16, con
     if (c.fused) {
         e = launch_conv_rows_fp16(c.log2n, c.corr, x16, y16, c.rows, b.spec, c.tw_rows, c.scale, st);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "k_conv_rows_h launch: %s", hipGetErrorString(e));
-    } else {
-        int cur = 0;
-        const void *src = b.scratch[1];  // not read by the first pass (it reads x16)
-        auto run = [&](const PassStep &s, int dir) -> int {
-            PassArgs a = s.args;
-            a.rin16 = x16;
-            a.rout16 = y16;
-            a.spectrum = b.spec;
-            void *dst = b.scratch[cur];
-            hipError_t le = dir ? launch_pass_fp16(s.log2b, s.log2t, dir, src, dst, a, st)
-                                : launch_pass_mid_fp16(s.log2b, s.log2t, src, dst, a, st);
-            if (le != hipSuccess) return set_error(ESTL_ERR_HIP, "FP16 conv pass launch: %s", hipGetErrorString(le));
-            src = dst;
-            cur ^= 1;
-            return ESTL_OK;
-        };
-        for (const auto &s : c.fwd)
-            if (int rc = run(s, ESTL_FORWARD)) return rc;
-        if (int rc = run(c.mid, 0)) return rc;
-        for (const auto &s : c.inv)
-            if (int rc = run(s, ESTL_BACKWARD)) return rc;
+    } else if (int rc = conv_pass_chain(c, x16, y16, b.spec, nullptr, 0, b.scratch[0], b.scratch[1], st)) {
+        return rc;
     }
     e = launch_conv_scales(s_in, s_out, b.spec + c.n, c.rows, st);
     if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "output scales: %s", hipGetErrorString(e));
