@@ -19,7 +19,8 @@
 //                   re / im parts go back to the two rows: one HBM read and one write
 //                   per value, one launch for the whole batch;
 //   * larger n runs the same pipeline as global passes with fused I/O (pass_tile.h);
-//   * k_conv_rows_h, k_filter_l1, k_spec_l1, k_conv_scales: the FP16-storage variant.
+//   * k_conv_rows on binary16 rows, k_filter_l1, k_spec_l1, k_conv_scales: the FP16-storage
+//                   variant (and group forms of the last three for request groups).
 #include "fft_block.h"
 #include "internal.h"
 
@@ -97,12 +98,21 @@ hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift,
     return hipGetLastError();
 }
 
+// real rows stored as FP32 (streaming loads / stores) or binary16 (FP32 arithmetic)
+__device__ __forceinline__ float ld_real(const float *p) { return ld_stream_f(p); }
+__device__ __forceinline__ float ld_real(const _Float16 *p) { return (float)*p; }
+__device__ __forceinline__ void st_real(float *p, float v) { st_stream_f(p, v); }
+__device__ __forceinline__ void st_real(_Float16 *p, float v) { *p = (_Float16)v; }
+
 // Persistent over pairs of real rows (TPW pairs per workgroup), like k_fft_rows.  With
 // a request group, `rows` counts the group's pairs twice and each slot takes its pair's
-// member: input rows, spectrum and output rows.
-template <int LOG2N, int TPW, bool CORR>
+// member: input rows, spectrum and output rows.  R: the stored real type, float or
+// _Float16 (FP16 storage: the spectrum has been divided by the filter's L1 norm, so
+// every stored output lies in [-1, 1] (|y| <= sum |h| max |x|); the caller's per-row
+// scales carry the magnitudes).
+template <int LOG2N, int TPW, bool CORR, typename R>
 __global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT * TPW)
-k_conv_rows(const float *__restrict__ x, float *__restrict__ y, long rows, const float2 *__restrict__ spec,
+k_conv_rows(const R *__restrict__ x, R *__restrict__ y, long rows, const float2 *__restrict__ spec,
             const float2 *__restrict__ tw, float scale, const ConvGroup *__restrict__ group) {
     using G = BlockGeom<LOG2N>;
     constexpr int N = G::N, E = G::E, TPT = G::TPT;
@@ -121,16 +131,16 @@ k_conv_rows(const float *__restrict__ x, float *__restrict__ y, long rows, const
             const int q = active ? conv_group_member(group, pr) : 0;
             r1 = active ? 2 * (pr - group->pair_start[q]) : 0;
             nrows = group->real_rows[q];
-            x = static_cast<const float *>(group->x[q]);
-            y = static_cast<float *>(group->y[q]);
+            x = static_cast<const R *>(group->x[q]);
+            y = static_cast<R *>(group->y[q]);
             spec = group->spec[q];
         }
         const bool two = active && r1 + 1 < nrows;
-        const float *x1 = x + r1 * N + tl;
+        const R *x1 = x + r1 * N + tl;
         float2 v[E];
 #pragma unroll
         for (int i = 0; i < E; ++i)
-            v[i] = make_float2(ld_stream_f(x1 + TPT * i), two ? ld_stream_f(x1 + N + TPT * i) : 0.0f);
+            v[i] = make_float2(ld_real(x1 + TPT * i), two ? ld_real(x1 + N + TPT * i) : 0.0f);
         block_fft<LOG2N, -1, kLdsPad>(v, lds + w * SLICE, tl, tw, true);
 #pragma unroll
         for (int i = 0; i < E; ++i) {
@@ -139,72 +149,20 @@ k_conv_rows(const float *__restrict__ x, float *__restrict__ y, long rows, const
         }
         block_fft<LOG2N, +1, kLdsPad>(v, lds + w * SLICE, tl, tw, true);
         if (active) {
-            float *y1 = y + r1 * N + tl;
+            R *y1 = y + r1 * N + tl;
 #pragma unroll
             for (int i = 0; i < E; ++i) {
-                st_stream_f(y1 + TPT * i, v[i].x * scale);
-                if (two) st_stream_f(y1 + N + TPT * i, v[i].y * scale);
+                st_real(y1 + TPT * i, v[i].x * scale);
+                if (two) st_real(y1 + N + TPT * i, v[i].y * scale);
             }
         }
     }
 }
 
-// FP16 storage: the same pipeline on binary16 real rows (FP32 arithmetic).  The
-// spectrum has been divided by the filter's L1 norm, so every stored output lies in
-// [-1, 1] (|y| <= sum |h| max |x|); the caller's per-row scales carry the magnitudes.
-template <int LOG2N, int TPW, bool CORR>
-__global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT * TPW)
-k_conv_rows_h(const _Float16 *__restrict__ x, _Float16 *__restrict__ y, long rows, const float2 *__restrict__ spec,
-              const float2 *__restrict__ tw, float scale, const ConvGroup *__restrict__ group) {
-    using G = BlockGeom<LOG2N>;
-    constexpr int N = G::N, E = G::E, TPT = G::TPT;
-    constexpr int SLICE = G::uses_lds ? Lds<kLdsPad>::size(N) : 0;
-    constexpr int LDS = G::uses_lds ? TPW * SLICE : 1;
-    __shared__ float2 lds[LDS];
-    const long pairs = (rows + 1) / 2;
-    const long groups = (pairs + TPW - 1) / TPW;
-    for (long grp = blockIdx.x; grp < groups; grp += gridDim.x) {  // uniform across the workgroup
-        const int t = opaque_lane(threadIdx.x);
-        const int w = t / TPT, tl = t % TPT;
-        const long pr = grp * TPW + w;
-        const bool active = pr < pairs;
-        long r1 = active ? 2 * pr : 0, nrows = rows;
-        if (group) {  // the pair's member, as in k_conv_rows
-            const int q = active ? conv_group_member(group, pr) : 0;
-            r1 = active ? 2 * (pr - group->pair_start[q]) : 0;
-            nrows = group->real_rows[q];
-            x = static_cast<const _Float16 *>(group->x[q]);
-            y = static_cast<_Float16 *>(group->y[q]);
-            spec = group->spec[q];
-        }
-        const bool two = active && r1 + 1 < nrows;
-        const _Float16 *x1 = x + r1 * N + tl;
-        float2 v[E];
-#pragma unroll
-        for (int i = 0; i < E; ++i)
-            v[i] = make_float2((float)x1[TPT * i], two ? (float)x1[N + TPT * i] : 0.0f);
-        block_fft<LOG2N, -1, kLdsPad>(v, lds + w * SLICE, tl, tw, true);
-#pragma unroll
-        for (int i = 0; i < E; ++i) {
-            const float2 hk = spec[tl + TPT * i];
-            v[i] = CORR ? cmulc(v[i], hk) : cmul(v[i], hk);
-        }
-        block_fft<LOG2N, +1, kLdsPad>(v, lds + w * SLICE, tl, tw, true);
-        if (active) {
-            _Float16 *y1 = y + r1 * N + tl;
-#pragma unroll
-            for (int i = 0; i < E; ++i) {
-                y1[TPT * i] = (_Float16)(v[i].x * scale);
-                if (two) y1[N + TPT * i] = (_Float16)(v[i].y * scale);
-            }
-        }
-    }
-}
-
-// l1->y = sum |h_t| over the k filter taps (one workgroup, fixed summation order, so
-// the value is the same on every call); it bounds the L1 norm of the folded filter.
-__global__ void __launch_bounds__(256) k_filter_l1(const float *__restrict__ h, long k, float2 *__restrict__ l1) {
-    __shared__ float part[256];
+// sum |h_t| over k filter taps by one 256-thread workgroup in a fixed summation order
+// (the same value on every call); valid in thread 0.  It bounds the L1 norm of the
+// folded filter.
+__device__ __forceinline__ float filter_l1(const float *__restrict__ h, long k, float *part) {
     float acc = 0.0f;
     for (long i = threadIdx.x; i < k; i += 256) acc += fabsf(h[i]);
     part[threadIdx.x] = acc;
@@ -213,7 +171,14 @@ __global__ void __launch_bounds__(256) k_filter_l1(const float *__restrict__ h, 
         if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) l1->y = part[0];
+    return part[0];
+}
+
+// l1->y = sum |h_t| over the k filter taps
+__global__ void __launch_bounds__(256) k_filter_l1(const float *__restrict__ h, long k, float2 *__restrict__ l1) {
+    __shared__ float part[256];
+    const float v = filter_l1(h, k, part);
+    if (threadIdx.x == 0) l1->y = v;
 }
 
 // spectrum /= L, L = l1->y (divided by |K^[0]| when the spectrum was normalised by the
@@ -255,18 +220,9 @@ hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1,
 // kept in g->l1[q].
 __global__ void __launch_bounds__(256) k_filter_l1_group(ConvGroup *__restrict__ g) {
     const int q = blockIdx.x;
-    const float *h = g->h[q];
-    const long k = g->k[q];
     __shared__ float part[256];
-    float acc = 0.0f;
-    for (long i = threadIdx.x; i < k; i += 256) acc += fabsf(h[i]);
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) g->l1[q].y = part[0];
+    const float v = filter_l1(g->h[q], g->k[q], part);
+    if (threadIdx.x == 0) g->l1[q].y = v;
 }
 
 __global__ void k_spec_l1_group(ConvGroup *__restrict__ g, long n) {
@@ -312,7 +268,7 @@ static hipError_t launch_conv_rows_t(const float *x, float *y, long rows, const 
                                      float scale, hipStream_t st, const ConvGroup *group) {
     using G = BlockGeom<LOG2N>;
     constexpr int TPW = (G::TPT >= 256) ? 1 : 256 / G::TPT;
-    auto kernel = k_conv_rows<LOG2N, TPW, CORR>;
+    auto kernel = k_conv_rows<LOG2N, TPW, CORR, float>;
     const long resident = conv_resident(kernel, G::TPT * TPW);
     const long pairs = (rows + 1) / 2;
     long blocks = (pairs + TPW - 1) / TPW;
@@ -326,7 +282,7 @@ static hipError_t launch_conv_rows_h_t(const void *x, void *y, long rows, const 
                                        float scale, hipStream_t st, const ConvGroup *group) {
     using G = BlockGeom<LOG2N>;
     constexpr int TPW = (G::TPT >= 256) ? 1 : 256 / G::TPT;
-    auto kernel = k_conv_rows_h<LOG2N, TPW, CORR>;
+    auto kernel = k_conv_rows<LOG2N, TPW, CORR, _Float16>;
     const long resident = conv_resident(kernel, G::TPT * TPW);
     const long pairs = (rows + 1) / 2;
     long blocks = (pairs + TPW - 1) / TPW;

@@ -294,7 +294,7 @@ int conv_run_group_fp16(const ConvShape &c, ConvGroup *d_group, int members, lon
     if (c.fused) {
         e = launch_conv_rows_fp16(c.log2n, c.corr, nullptr, nullptr, 2 * pairs, nullptr, c.tw_rows, c.scale, st,
                                   d_group);
-        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group k_conv_rows_h launch: %s", hipGetErrorString(e));
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group FP16 k_conv_rows launch: %s", hipGetErrorString(e));
     } else if (int rc = conv_pass_chain(c, nullptr, nullptr, nullptr, d_group, pairs, scratch0, scratch1, st)) {
         return rc;
     }
@@ -310,7 +310,7 @@ int conv_run_fp16(const ConvShape &c, const ConvBuffers &b, const void *x16, con
     hipError_t e;
     if (c.fused) {
         e = launch_conv_rows_fp16(c.log2n, c.corr, x16, y16, c.rows, b.spec, c.tw_rows, c.scale, st);
-        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "k_conv_rows_h launch: %s", hipGetErrorString(e));
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "FP16 k_conv_rows launch: %s", hipGetErrorString(e));
     } else if (int rc = conv_pass_chain(c, x16, y16, b.spec, nullptr, 0, b.scratch[0], b.scratch[1], st)) {
         return rc;
     }
