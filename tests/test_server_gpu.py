@@ -386,3 +386,36 @@ def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags):
     assert st["failed"] == 0 and st["groups"] >= 1 and st["grouped_requests"] >= 3, st
     assert st["arena_used_bytes"] == 0
     srv.close()
+
+
+def test_request_groups_at_member_limit(gpu):
+    """Forty queued requests of one length form several groups of at most 16 members
+    (the descriptor's limit); every result stays bit-identical to the request alone."""
+    import torch
+    n, count = 2 ** 14, 40
+    rng = np.random.default_rng(77)
+    rows_list = [int(r) for r in rng.integers(1, 9, size=count)]
+    ks = [int(k) for k in rng.integers(1, 2000, size=count)]
+    cases = [("conv", n, r, k, estl_amd.CONV_CORRELATE if i % 3 == 0 else 0)
+             for i, (r, k) in enumerate(zip(rows_list, ks))]
+    inputs = [make_inputs(torch, gpu, c, 700 + i) for i, c in enumerate(cases)]
+    refs = [alone(torch, gpu, c, x, h) for c, (x, h) in zip(cases, inputs)]
+    srv = estl_amd.Server(4 << 30, streams=1)
+    side = _hold(torch)
+    ids = []
+    for i, c in enumerate(cases):
+        kind, n_, rows, k, fl = c
+        x, h = inputs[i]
+        ids.append((i, srv.submit_conv(n_, rows, x, h, k, fl, stream=side if i < 2 else None)))
+    srv.drain(120)
+    for i, rid in ids:
+        srv.wait(rid, 60)
+        assert torch.equal(result(srv, rid, cases[i], torch), refs[i]), (i, rid)
+        srv.release(rid)
+    st = srv.stats()
+    assert st["failed"] == 0 and st["completed"] == count
+    # two flag classes among 38 queued requests: at least three groups, none over 16
+    assert st["groups"] >= 3 and st["grouped_requests"] >= 30, st
+    assert st["grouped_requests"] <= 16 * st["groups"], st
+    assert st["arena_used_bytes"] == 0
+    srv.close()
