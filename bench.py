@@ -212,20 +212,28 @@ def _drive(srv, reqs, xp, hp, arrivals, filters=None, keep=(), fp16=None):
     (hp[:k], spectrum computed inside the request) or, when `filters` maps (n, bank
     index) to a registered filter, with that -- and release each output as soon as the
     completion queue reports it; outputs of the indices in `keep` are held and returned."""
+    import torch
     held, where = {}, {}
     keep = set(keep)
     i, count = 0, len(reqs.n)
+    # the client holds its requests, its buffers' device addresses and its stream (the
+    # pools are sized for the largest request)
+    args = list(zip(reqs.n.tolist(), reqs.rows.tolist(), reqs.k.tolist(), reqs.flags.tolist()))
+    banks = reqs.bank.tolist()
+    px, ph = xp.data_ptr(), hp.data_ptr()
+    p16 = (fp16[0].data_ptr(), fp16[1].data_ptr()) if fp16 is not None else None
+    ps = torch.cuda.current_stream().cuda_stream
     t0 = time.perf_counter()
     while i < count or where:
         now = time.perf_counter() - t0
         while i < count and arrivals[i] <= now:
-            n, rows, k, fl = int(reqs.n[i]), int(reqs.rows[i]), int(reqs.k[i]), int(reqs.flags[i])
-            if fp16 is not None:        # (binary16 input pool, per-row scales)
-                rid = srv.submit_conv_fp16(n, rows, fp16[0], fp16[1], hp, k, fl)
+            n, rows, k, fl = args[i]
+            if p16 is not None:        # (binary16 input pool, per-row scales)
+                rid = srv.submit_conv_fp16(n, rows, p16[0], p16[1], ph, k, fl, stream=ps)
             elif filters is not None:
-                rid = srv.submit_conv_filter(filters[(n, int(reqs.bank[i]))], rows, xp, fl)
+                rid = srv.submit_conv_filter(filters[(n, banks[i])], rows, px, fl, stream=ps)
             else:
-                rid = srv.submit_conv(n, rows, xp, hp, k, fl)
+                rid = srv.submit_conv(n, rows, px, ph, k, fl, stream=ps)
             where[rid] = i
             i += 1
         wait = max(0.0, arrivals[i] - (time.perf_counter() - t0)) if i < count else 0.05
