@@ -238,10 +238,16 @@ __global__ void k_spec_l1_group(ConvGroup *__restrict__ g, long n) {
     if (i == 0) g->l1[q].x = L;
 }
 
-__global__ void k_conv_scales_group(const ConvGroup *__restrict__ g) {
+// output row scales of every member, s_out = s_in * L: L from the descriptor (inline
+// filters, written by k_spec_l1_group) or from the value after the member's n spectrum
+// values (registered filters, written by k_spec_l1 when the filter was registered)
+__global__ void k_conv_scales_group(const ConvGroup *__restrict__ g, long n, int registered) {
     const int q = blockIdx.y;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < g->real_rows[q]) g->yscale[q][i] = g->xscale[q][i] * g->l1[q].x;
+    if (i < g->real_rows[q]) {
+        const float L = registered ? g->spec[q][n].x : g->l1[q].x;
+        g->yscale[q][i] = g->xscale[q][i] * L;
+    }
 }
 
 hipError_t launch_conv_fp16_filter_group(ConvGroup *g, int members, long n, hipStream_t st) {
@@ -251,10 +257,11 @@ hipError_t launch_conv_fp16_filter_group(ConvGroup *g, int members, long n, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_conv_scales_group(const ConvGroup *g, int members, long max_rows, hipStream_t st) {
+hipError_t launch_conv_scales_group(const ConvGroup *g, int members, long max_rows, long n, bool registered,
+                                    hipStream_t st) {
     if (members < 1 || members > kGroupMax || max_rows < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_conv_scales_group, dim3((unsigned)((max_rows + 255) / 256), (unsigned)members), dim3(256), 0,
-                       st, g);
+                       st, g, n, registered ? 1 : 0);
     return hipGetLastError();
 }
 

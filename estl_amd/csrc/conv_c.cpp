@@ -32,7 +32,8 @@ hipError_t launch_spec_normalize(float2 *spec, long n, const float2 *dc, hipStre
 hipError_t launch_conv_rows_fp16(int log2n, bool corr, const void *x, void *y, long rows, const float2 *spec,
                                  const float2 *tw, float scale, hipStream_t st, const ConvGroup *group = nullptr);
 hipError_t launch_conv_fp16_filter_group(ConvGroup *g, int members, long n, hipStream_t st);
-hipError_t launch_conv_scales_group(const ConvGroup *g, int members, long max_rows, hipStream_t st);
+hipError_t launch_conv_scales_group(const ConvGroup *g, int members, long max_rows, long n, bool registered,
+                                    hipStream_t st);
 hipError_t launch_conv_fp16_filter(const float *h, long k, float2 *spec, long n, const float2 *dc, bool normalized,
                                    hipStream_t st);
 hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1, long rows, hipStream_t st);
@@ -279,7 +280,9 @@ int conv_run_group(const ConvShape &c, const ConvGroup *d_group, long pairs, flo
 }
 
 // FP16 storage: every member's spectrum divided by its filter's L = sum |h| (kept in
-// the descriptor), the passes on binary16 rows, then the members' output row scales.
+// the descriptor; a registered filter's spectrum was divided when it was registered,
+// with L after its n values), the passes on binary16 rows, then the members' output
+// row scales.
 int conv_fp16_filter_group(ConvGroup *d_group, int members, long n, hipStream_t st) {
     hipError_t e = launch_conv_fp16_filter_group(d_group, members, n, st);
     if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group FP16 filter scaling: %s", hipGetErrorString(e));
@@ -287,7 +290,7 @@ int conv_fp16_filter_group(ConvGroup *d_group, int members, long n, hipStream_t 
 }
 
 int conv_run_group_fp16(const ConvShape &c, ConvGroup *d_group, int members, long pairs, long max_rows,
-                        void *scratch0, void *scratch1, hipStream_t st) {
+                        bool registered, void *scratch0, void *scratch1, hipStream_t st) {
     if (c.precision != ESTL_PRECISION_FP16) return set_error(ESTL_ERR_INVALID, "not an FP16-storage shape");
     if (pairs < 1) return set_error(ESTL_ERR_INVALID, "empty request group");
     hipError_t e;
@@ -298,7 +301,7 @@ int conv_run_group_fp16(const ConvShape &c, ConvGroup *d_group, int members, lon
     } else if (int rc = conv_pass_chain(c, nullptr, nullptr, nullptr, d_group, pairs, scratch0, scratch1, st)) {
         return rc;
     }
-    e = launch_conv_scales_group(d_group, members, max_rows, st);
+    e = launch_conv_scales_group(d_group, members, max_rows, c.n, registered, st);
     if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group output scales: %s", hipGetErrorString(e));
     return ESTL_OK;
 }

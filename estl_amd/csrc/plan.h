@@ -119,8 +119,10 @@ int group_spectrum_plan(Plan *p, long n, long filters);
 int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectra, float2 *scratch0, hipStream_t st);
 int conv_filter_group_rows(const ConvShape &c, const ConvGroup *d_group, int members, hipStream_t st);
 int conv_fp16_filter_group(ConvGroup *d_group, int members, long n, hipStream_t st);
+// registered: the members' spectra are registered filters' (already divided by L, which
+// follows their n values)
 int conv_run_group_fp16(const ConvShape &c, ConvGroup *d_group, int members, long pairs, long max_rows,
-                        void *scratch0, void *scratch1, hipStream_t st);
+                        bool registered, void *scratch0, void *scratch1, hipStream_t st);
 int conv_run_group(const ConvShape &c, const ConvGroup *d_group, long pairs, float2 *scratch0, float2 *scratch1,
                    hipStream_t st);
 int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y, hipStream_t st);

@@ -430,12 +430,10 @@ class Server {
         return raw;
     }
 
-    // a convolution whose filter is ready (inline, or registered and done; FP16 storage
-    // with inline filters only: a group keeps their L1 norms in its descriptor)
+    // a convolution whose filter is ready (inline, or registered and done)
     bool groupable(const Request &r) const {
         if (r.kind != KIND_CONV && r.kind != KIND_CONV_REG) return false;
         if (r.flags & ESTL_CONV_NORMALIZE_KERNEL) return false;
-        if ((r.flags & ESTL_CONV_FP16) && r.kind != KIND_CONV) return false;
         if (r.n < 2 || r.rows < 1) return false;
         if (r.kind == KIND_CONV_REG) {
             auto f = reqs_.find(r.filter);
@@ -596,8 +594,9 @@ class Server {
             if (int rc = conv_filter_group(*r.gspec, d_desc, spectra, scr0, st)) return rc;
         }
         if (fp16) {
-            if (int rc = conv_fp16_filter_group(d_desc, g.count, n, st)) return rc;
-            return conv_run_group_fp16(*r.gshape, d_desc, g.count, pair, max_rows, scr0, scr1, st);
+            if (inline_filters)
+                if (int rc = conv_fp16_filter_group(d_desc, g.count, n, st)) return rc;
+            return conv_run_group_fp16(*r.gshape, d_desc, g.count, pair, max_rows, !inline_filters, scr0, scr1, st);
         }
         return conv_run_group(*r.gshape, d_desc, pair, scr0, scr1, st);
     }

@@ -254,10 +254,9 @@ int estl_conv_destroy(estl_conv_t conv);
  * ESTL_ERR_BUSY.  The server's streams wait for the work queued on `input_stream` at
  * submit (the producer of the inputs); input buffers must stay valid until the request
  * is DONE.  Request groups: a convolution (no kernel normalisation; a registered
- * filter must be ready; with FP16 storage, inline filters only) admitted from the head of the
- * queue takes along queued requests of the same kind, n and flags from the next 256
- * queue entries (at most 16 requests, 2^24 values), which are thereby admitted ahead of
- * requests between them; the group runs as one batch and its members complete
+ * filter must be ready) admitted from the head of the queue takes along queued
+ * requests of the same kind, n and flags from the next 256 queue entries (at most 16
+ * requests, 2^24 values), which are thereby admitted ahead of requests between them; the group runs as one batch and its members complete
  * together.  Each member keeps its id, output and release.  Results are bit-identical
  * to the same request served alone, grouped or not. */
 #define ESTL_REQ_QUEUED 0

@@ -346,11 +346,16 @@ def test_request_groups_equal_alone_bitwise(gpu, n, flags):
     srv.close()
 
 
-@pytest.mark.parametrize("n,flags", [(2 ** 16, estl_amd.CONV_CORRELATE), (2 ** 11, estl_amd.CONV_CENTER)])
-def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags):
+@pytest.mark.parametrize("n,flags,registered", [
+    (2 ** 16, estl_amd.CONV_CORRELATE, False), (2 ** 11, estl_amd.CONV_CENTER, False),
+    (2 ** 16, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER, True), (2 ** 12, estl_amd.CONV_CENTER, True),
+])
+def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags, registered):
     """FP16-storage requests of one length, queued behind a busy input stream, run as a
-    group (the filters' L1 norms kept in the group's descriptor); each binary16 output
-    and its row scales are bit-identical to the FP16 convolution object's."""
+    group: with inline filters the filters' L1 norms are kept in the group's descriptor,
+    with registered filters (centring a filter option, correlation a request option) L
+    follows each filter's spectrum.  Each binary16 output and its row scales are
+    bit-identical to the FP16 convolution object's."""
     import torch
     rows_list = [3, 1, 8, 5, 2, 7]
     ks = [100, 16, 2000, 33, 8192, 1]
@@ -371,9 +376,21 @@ def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags):
         refs.append((y16.clone(), s_out.clone()))
         keep.append((x16, s_in, h))
     srv = estl_amd.Server(1 << 30, streams=1)
+    fids = []
+    if registered:
+        fflags = (flags & estl_amd.CONV_CENTER) | estl_amd.CONV_FP16
+        fids = [srv.register_filter(n, h, int(h.numel()), fflags) for (_, _, h) in keep]
+        for f in fids:
+            srv.wait(f, 60)
     side = _hold(torch)
-    ids = [srv.submit_conv_fp16(n, rows, x16, s_in, h, int(h.numel()), flags, stream=side if i < 2 else None)
-           for i, (rows, (x16, s_in, h)) in enumerate(zip(rows_list, keep))]
+    ids = []
+    for i, (rows, (x16, s_in, h)) in enumerate(zip(rows_list, keep)):
+        st = side if i < 2 else None
+        if registered:
+            ids.append(srv.submit_conv_filter_fp16(fids[i], rows, x16, s_in, flags & estl_amd.CONV_CORRELATE,
+                                                   stream=st))
+        else:
+            ids.append(srv.submit_conv_fp16(n, rows, x16, s_in, h, int(h.numel()), flags, stream=st))
     srv.drain(120)
     for i, rid in enumerate(ids):
         srv.wait(rid, 60)
@@ -384,7 +401,9 @@ def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags):
         srv.release(rid)
     st = srv.stats()
     assert st["failed"] == 0 and st["groups"] >= 1 and st["grouped_requests"] >= 3, st
-    assert st["arena_used_bytes"] == 0
+    for f in fids:
+        srv.release(f)
+    assert srv.stats()["arena_used_bytes"] == 0
     srv.close()
 
 
