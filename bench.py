@@ -34,6 +34,36 @@ SEED = 1
 METRIC = "batched C2C fp32 GB/s & % HBM peak vs per-call baseline; p50/p99 latency"
 
 
+_ROCTX = []
+
+
+def leg(name: str):
+    """A named range around one leg of the benchmark, recorded by a profiler run with
+    `rocprofv3 --marker-trace` through rocprofiler-sdk's roctx library (nothing is
+    recorded, and nothing changes, when that library is absent)."""
+    import contextlib
+    import ctypes
+    if not _ROCTX:
+        try:
+            lib = ctypes.CDLL("/opt/rocm/lib/librocprofiler-sdk-roctx.so")
+            lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+            _ROCTX.append(lib)
+        except OSError:
+            _ROCTX.append(None)
+
+    @contextlib.contextmanager
+    def rng():
+        lib = _ROCTX[0]
+        if lib is not None:
+            lib.roctxRangePushA(name.encode())
+        try:
+            yield
+        finally:
+            if lib is not None:
+                lib.roctxRangePop()
+    return rng()
+
+
 def percentile(xs, p):
     xs = sorted(xs)
     if not xs:
@@ -277,7 +307,8 @@ def bench_serve(torch, estl_amd, stream, load: float = 0.7, budget: int = 16 << 
     srv.next_done(4096, 0.0)
     # capacity: everything submitted at t = 0
     srv.reset_stats()
-    wall_cap, _ = _drive(srv, reqs, xp, hp, [0.0] * count)
+    with leg("serve.capacity"):
+        wall_cap, _ = _drive(srv, reqs, xp, hp, [0.0] * count)
     cap = srv.stats()
     capacity = count / wall_cap
     # open loop: Poisson arrivals at load x capacity
@@ -285,7 +316,8 @@ def bench_serve(torch, estl_amd, stream, load: float = 0.7, budget: int = 16 << 
     arrivals = (reqs.gaps / rate).cumsum()
     sample = [0, 1, 2, 500, 999]
     srv.reset_stats()
-    wall, held = _drive(srv, reqs, xp, hp, arrivals, keep=sample)
+    with leg("serve.load"):
+        wall, held = _drive(srv, reqs, xp, hp, arrivals, keep=sample)
     st = srv.stats()
     same = True
     for j, rid in held.items():
@@ -305,7 +337,8 @@ def bench_serve(torch, estl_amd, stream, load: float = 0.7, budget: int = 16 << 
     for f in filters.values():
         srv.wait(f, 60)
     srv.reset_stats()
-    wall_bank, _ = _drive(srv, reqs, xp, hp, [0.0] * count, filters)
+    with leg("serve.filter_bank"):
+        wall_bank, _ = _drive(srv, reqs, xp, hp, [0.0] * count, filters)
     bank = srv.stats()
     # FP16 storage: the same stream on binary16 inputs (unit row scales: the pool is
     # already in [-1, 1)), outputs binary16 with per-row scales
@@ -318,7 +351,8 @@ def bench_serve(torch, estl_amd, stream, load: float = 0.7, budget: int = 16 << 
         srv.release(rid)
     srv.next_done(4096, 0.0)
     srv.reset_stats()
-    wall_cap16, _ = _drive(srv, reqs, xp, hp, [0.0] * count, fp16=(x16, ones))
+    with leg("serve.fp16_capacity"):
+        wall_cap16, _ = _drive(srv, reqs, xp, hp, [0.0] * count, fp16=(x16, ones))
     cap16 = srv.stats()
     rate16 = load * count / wall_cap16
     arrivals16 = (reqs.gaps / rate16).cumsum()
