@@ -38,6 +38,40 @@ hipError_t launch_conv_fp16_filter(const float *h, long k, float2 *spec, long n,
                                    hipStream_t st);
 hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1, long rows, hipStream_t st);
 
+// A filter whose k taps all lie within the first spectrum pass's stride n / B_1,
+// unrotated, puts one nonzero value in each column of that pass, whose DFT is that value
+// in every output: the pass is skipped and the second reads the taps (io_in = 4).
+bool spectrum_skips_first_pass(const Plan &spec, long k, long shift) {
+    if (spec.kind != Plan::PASSES || spec.steps.size() < 2 || shift != 0 || k < 1) return false;
+    return k <= (spec.n >> spec.steps.front().log2b);
+}
+
+// The passes of a spectrum plan over the filter(s) given by `a0` (taps, count, rotation,
+// or a group descriptor): the first reads the taps (io_in = 3), or, when `skip`, the
+// second does (io_in = 4); destinations alternate back from `spec` with `scratch0`.
+static int spectrum_passes(const Plan &plan, const PassArgs &taps, bool skip, float2 *spec, float2 *scratch0,
+                           hipStream_t st) {
+    const auto &steps = plan.steps;
+    const int P = (int)steps.size();
+    const int q0 = skip ? 1 : 0;
+    const float2 *src = scratch0;  // not read by the first pass run (it reads the taps)
+    for (int q = q0; q < P; ++q) {
+        PassArgs a = steps[q].args;
+        if (q == q0) {
+            a.io_in = skip ? 4 : 3;
+            a.rin = taps.rin;
+            a.fk = taps.fk;
+            a.fshift = taps.fshift;
+        }
+        a.group = taps.group;
+        float2 *dst = ((P - 1 - q) & 1) ? scratch0 : spec;
+        hipError_t e = launch_pass(steps[q].log2b, steps[q].log2t, ESTL_FORWARD, src, dst, a, st);
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "filter spectrum pass: %s", hipGetErrorString(e));
+        src = dst;
+    }
+    return ESTL_OK;
+}
+
 int group_spectrum_plan(Plan *p, long n, long filters) {
     int rc = build_plan(p, n, filters, ESTL_FORWARD, ESTL_PRECISION_FP32, 0, nullptr);
     if (rc) return rc;
@@ -138,24 +172,15 @@ int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k
         e = launch_filter_spectrum(c.log2n, h, k, shift, b.spec, c.spec_plan.tw_rows, normalize, st);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "filter spectrum launch: %s", hipGetErrorString(e));
     } else {
-        // the spectrum plan's passes, the first building the filter from its taps; the
-        // destinations alternate back from the spectrum with scratch 0
+        // the spectrum plan's passes, the first run building the filter from its taps
         if (!b.scratch[0]) return set_error(ESTL_ERR_INVALID, "the filter spectrum of n=%ld needs scratch", c.n);
-        const auto &steps = c.spec_plan.steps;
-        const int P = (int)steps.size();
-        const float2 *src = b.scratch[0];  // not read by the first pass (it reads h)
-        for (int q = 0; q < P; ++q) {
-            PassArgs a = steps[q].args;
-            if (q == 0) {
-                a.rin = h;
-                a.fk = k;
-                a.fshift = shift;
-            }
-            float2 *dst = ((P - 1 - q) & 1) ? b.scratch[0] : b.spec;
-            e = launch_pass(steps[q].log2b, steps[q].log2t, ESTL_FORWARD, src, dst, a, st);
-            if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "filter spectrum pass: %s", hipGetErrorString(e));
-            src = dst;
-        }
+        PassArgs taps{};
+        taps.rin = h;
+        taps.fk = k;
+        taps.fshift = shift;
+        if (int rc = spectrum_passes(c.spec_plan, taps, spectrum_skips_first_pass(c.spec_plan, k, shift), b.spec,
+                                     b.scratch[0], st))
+            return rc;
     }
     if (normalize && c.spec_plan.kind != Plan::ROWS) {
         // divide by the kernel sum K^[0] (Convolution.h:141): spectrum / spectrum[0].re
@@ -245,20 +270,14 @@ int conv_filter_group_rows(const ConvShape &c, const ConvGroup *d_group, int mem
     return ESTL_OK;
 }
 
-int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectra, float2 *scratch0, hipStream_t st) {
+int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectra, float2 *scratch0,
+                      bool skip_first, hipStream_t st) {
     if (spec.kind != Plan::PASSES || spec.steps.empty() || spec.steps.front().args.io_in != 3)
         return set_error(ESTL_ERR_INVALID, "not a group spectrum plan");
-    const int P = (int)spec.steps.size();
-    const float2 *src = scratch0;  // not read by the first pass (it reads the taps)
-    for (int q = 0; q < P; ++q) {
-        PassArgs a = spec.steps[q].args;
-        a.group = d_group;
-        float2 *dst = ((P - 1 - q) & 1) ? scratch0 : spectra;
-        hipError_t e = launch_pass(spec.steps[q].log2b, spec.steps[q].log2t, ESTL_FORWARD, src, dst, a, st);
-        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group spectrum pass: %s", hipGetErrorString(e));
-        src = dst;
-    }
-    return ESTL_OK;
+    if (skip_first && spec.steps.size() < 2) return set_error(ESTL_ERR_INVALID, "no first pass to skip");
+    PassArgs taps{};
+    taps.group = d_group;
+    return spectrum_passes(spec, taps, skip_first, spectra, scratch0, st);
 }
 
 // The convolution passes of a request group over its `pairs` pairs (the members' pairs

@@ -51,6 +51,11 @@ struct PassArgs {
     //   io_in  = 3: the row is a convolution filter built from its fk real taps at rin:
     //               folded modulo n and rotated by fshift (fold_tap); first pass of the
     //               filter's spectrum plan, direct load path (S >= T)
+    //   io_in  = 4: the row is what a first pass of radix A makes of such a filter when
+    //               its fk taps all lie within that pass's stride n / A, unrotated: its
+    //               DFT of one nonzero value per column is that value in every output,
+    //               so element e is tap e mod (n / A) (0 past fk); the second pass of the
+    //               spectrum plan, which then skips its first pass (either load path)
     int io_in, io_out, corr;
     long real_rows;          // real rows behind the pairs (the last pair may be single)
     const float *rin;
@@ -60,7 +65,7 @@ struct PassArgs {
     int planar_in;           // `in` / `out` use the tile-planarized layout (FP32 only)
     int planar_out;
     const float2 *spectrum;
-    long fk, fshift;         // io_in = 3: taps and rotation of the filter
+    long fk, fshift;         // io_in = 3, 4: taps and rotation of the filter
     // convolution passes of a request group: rows index the group's batch, and the
     // real-pair I/O, the filter loads and the middle pass's spectrum are the member's
     const ConvGroup *group;

@@ -119,6 +119,24 @@ __device__ __forceinline__ void pass_tile_load(const ST *__restrict__ in, const 
 #pragma unroll
         for (int i = 0; i < E; ++i)
             v[i] = make_float2(fold_tap(h, fk, n, fsh, base + ((P.tl + TPT * i) << ls)), 0.0f);
+    } else if (F32 && a.io_in == 4) {
+        // the first spectrum pass's output, rebuilt from the taps (the host checks that
+        // every tap lies within the first pass's stride n / A): element e is tap
+        // e mod (n / A), read at the element indices of the direct or the staged layout
+        const float *h = a.rin;
+        long fk = a.fk;
+        if (a.group) {
+            h = a.group->h[P.row];
+            fk = a.group->k[P.row];
+        }
+        const int mask = int(((1L << a.log2n) / a.A) - 1);
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int e = ls >= LOG2T ? (P.g << (ls + LOG2B)) + P.z + ((P.tl + TPT * i) << ls)
+                                      : (P.j0 << LOG2B) + P.t + NT * i;
+            const int m = e & mask;
+            v[i] = make_float2(m < fk ? h[m] : 0.0f, 0.0f);
+        }
     } else if (F32 && a.planar_in) {
         // the same values from the tile-planarized layout, by element index
         const long e0 = (P.row << a.log2n) +
@@ -165,7 +183,7 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
     const int lcpr = a.log2n - LOG2B;
     const int c = P.c, tl = P.tl, j = P.j;
     bool busy = lds_busy;
-    if (a.io_in != 1 && a.io_in != 3 && ls < LOG2T) {
+    if ((a.io_in == 0 || a.io_in == 4) && ls < LOG2T) {
         // value o of the contiguous block belongs to column (o / (S B)) S + o mod S,
         // element (o mod S B) / S
         const int sbmask = (1 << (ls + LOG2B)) - 1, smask = (1 << ls) - 1;

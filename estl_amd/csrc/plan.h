@@ -116,7 +116,10 @@ int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k
 // request groups (the server's continuous batching): the spectrum plan over `filters`
 // rows, the group's spectra and its convolution passes
 int group_spectrum_plan(Plan *p, long n, long filters);
-int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectra, float2 *scratch0, hipStream_t st);
+// skip_first: every member's filter passes spectrum_skips_first_pass (its k, its shift)
+bool spectrum_skips_first_pass(const Plan &spec, long k, long shift);
+int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectra, float2 *scratch0,
+                      bool skip_first, hipStream_t st);
 int conv_filter_group_rows(const ConvShape &c, const ConvGroup *d_group, int members, hipStream_t st);
 int conv_fp16_filter_group(ConvGroup *d_group, int members, long n, hipStream_t st);
 // registered: the members' spectra are registered filters' (already divided by L, which

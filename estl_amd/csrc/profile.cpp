@@ -138,7 +138,7 @@ int profile_plan(Plan *p, const void *in, void *out, int reps, std::string *json
     for (int q = 0; q < K; ++q) {
         const double f = plan_kernel_flops(p, q);
         snprintf(buf, sizeof buf,
-                 "%s{\"index\": %d, \"info\": \"%s\", \"ms\": %.6f, \"flops\": %.0f, \"gflops\": %.3f, \"share\": %.4f}",
+                 "%s{\"index\": %d, \"info\": \"%s\", \"ms\": %.6f, \"flops\": %.0f, \"gflops\": %.3f, \"share\": %.9f}",
                  q ? ", " : "", q, plan_kernel_info(p, q).c_str(), ms[q], f, ms[q] > 0 ? f / (ms[q] * 1e6) : 0.0,
                  total > 0 ? ms[q] / total : 0.0);
         s += buf;

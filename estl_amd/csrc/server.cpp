@@ -591,7 +591,9 @@ class Server {
             if (int rc = conv_filter_group_rows(*r.gshape, d_desc, g.count, st)) return rc;
         } else if (inline_filters) {
             if (r.gspec->batch != g.count) return set_error(ESTL_ERR_INVALID, "group spectrum plan size");
-            if (int rc = conv_filter_group(*r.gspec, d_desc, spectra, scr0, st)) return rc;
+            bool skip = true;  // the first spectrum pass is skipped when every filter allows it
+            for (int q = 0; q < g.count; ++q) skip = skip && spectrum_skips_first_pass(*r.gspec, g.k[q], g.shift[q]);
+            if (int rc = conv_filter_group(*r.gspec, d_desc, spectra, scr0, skip, st)) return rc;
         }
         if (fp16) {
             if (inline_filters)

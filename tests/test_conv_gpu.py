@@ -99,6 +99,31 @@ def test_long_filter_centred_and_normalised(gpu, n):
     check_rows(y, oracle.conv_circular(x.astype(np.float64), np.roll(hp, -s) / hp.sum()), n)
 
 
+@pytest.mark.parametrize("n,k,period", [
+    (2 ** 15, 256, 256), (2 ** 15, 100, 256), (2 ** 19, 8192, 2 ** 13), (2 ** 19, 1000, 2 ** 13),
+    (2 ** 20, 2 ** 14, 2 ** 14), (2 ** 22, 3000, 2 ** 15),
+])
+def test_spectrum_first_pass_skip_is_exact(gpu, n, k, period):
+    """A filter whose taps all lie within the first spectrum pass's stride (`period`,
+    n over the first radix of the default split) skips that pass: the second reads the
+    taps.  The same filter padded with zero taps to one past the stride takes the full
+    pass chain.  The two results are identical, and both match the convolution theorem
+    in FP64 (numpy's FFT of the folded filter); at 2^15 also the oracle's direct sum."""
+    import torch
+    rows = 2 if n <= 2 ** 19 else 1
+    x = oracle.fill_uniform(rows * n, 50).astype(np.float32).reshape(rows, n)
+    h = oracle.fill_uniform(k, 51).astype(np.float32)
+    hpad = np.concatenate([h, np.zeros(period + 1 - k, np.float32)])
+    y = conv_gpu(torch, gpu, x, h)
+    ypad = conv_gpu(torch, gpu, x, hpad)
+    assert np.array_equal(y, ypad)
+    x64 = x.astype(np.float64)
+    ref = np.real(np.fft.ifft(np.fft.fft(x64, axis=1) * np.fft.fft(folded(h.astype(np.float64), n))[None, :], axis=1))
+    check_rows(y, ref, n)
+    if n <= 2 ** 15:
+        check_rows(y, oracle.conv_circular(x64, h.astype(np.float64)), n)
+
+
 def test_identity_and_delay_closed_forms(gpu):
     import torch
     n, rows = 2 ** 14, 5

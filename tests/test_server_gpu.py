@@ -289,14 +289,17 @@ def _hold(torch):
 
 
 @pytest.mark.parametrize("n,flags", [(2 ** 15, 0), (2 ** 15, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER),
-                                     (2 ** 12, 0), (2 ** 13, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER)])
+                                     (2 ** 12, 0), (2 ** 13, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER),
+                                     (2 ** 19, 0)])
 def test_request_groups_equal_alone_bitwise(gpu, n, flags):
     """Queued convolutions of one length run as one group (one launch per pass, or one
     fused launch, for all of them); every member's result is bit-identical to the
     request run alone, with odd and even row counts, distinct inputs and filters
     (inline, then registered), and a request of another length queued in between.  One
     stream (two requests in flight) whose first two requests wait on a busy input
-    stream, so the rest queue up and group."""
+    stream, so the rest queue up and group.  At 2^15 with no rotation, the filters of
+    up to 256 taps skip the first spectrum pass alone but not in a group with longer
+    ones; at 2^19 every filter skips it, alone and grouped."""
     import torch
     rows_list = [1, 2, 3, 7, 16, 5, 1, 9, 4, 2]
     ks = [16, 300, 4097, 64, 2048, 1, 999, 8192, 33, 100]
