@@ -265,6 +265,52 @@ hipError_t launch_conv_scales_group(const ConvGroup *g, int members, long max_ro
     return hipGetLastError();
 }
 
+// One real row of n values convolved through complex transforms of M = n/2 values:
+// the row read as z[m] = x[2m] + i x[2m+1] has Z = DFT_M(z), and with k' = M - k
+// (indices of Z mod M) and W = exp(-2 pi i / n), for k in [0, M]:
+//   2 X[k] = (Z[k] + conj Z[k']) - i W^k (Z[k] - conj Z[k'])     (the row's DFT)
+//   2 Y[k] = 2 X[k] H[k]                    (H[k] conjugated for a correlation)
+//   4 V[k] = (2Y[k] + conj 2Y[k']) + i W^-k (2Y[k] - conj 2Y[k'])
+// and the inverse DFT_M of V (1/M) is y[2m] + i y[2m+1].  One thread per pair
+// {k, k'}, k in [0, M/2]; at k' the same terms appear conjugated (2 X[k'] = conj E -
+// i conj(W^k D) with E, D the sum and difference above, likewise for V).  V overwrites
+// Z in place, times 1/(2n), so the unnormalised inverse transform yields y.
+__global__ void __launch_bounds__(256) k_conv_half_merge(float2 *__restrict__ zv, const float2 *__restrict__ spec,
+                                                         long n, int corr) {
+    const long M = n >> 1;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > (M >> 1)) return;
+    const long kp = M - k;  // in [M/2, M]; Z[M] is Z[0]
+    const float2 zk = zv[k], zp = zv[kp == M ? 0 : kp];
+    double sd, cd;
+    sincospi(2.0 * (double)k / (double)n, &sd, &cd);
+    const float2 w = make_float2((float)cd, -(float)sd);  // W^k
+    const float2 e = make_float2(zk.x + zp.x, zk.y - zp.y);  // Z[k] + conj Z[k']
+    const float2 d = make_float2(zk.x - zp.x, zk.y + zp.y);  // Z[k] - conj Z[k']
+    const float2 a = cmul(d, w);                               // W^k D
+    // 2X[k] = E - i A, 2X[k'] = conj E - i conj A  (-i z = (z.y, -z.x))
+    const float2 xk = make_float2(e.x + a.y, e.y - a.x);
+    const float2 xp = make_float2(e.x - a.y, -e.y - a.x);
+    const float2 hk = spec[k], hp = spec[kp];
+    const float2 yk = corr ? cmulc(xk, hk) : cmul(xk, hk);
+    const float2 yp = corr ? cmulc(xp, hp) : cmul(xp, hp);
+    const float2 f = make_float2(yk.x + yp.x, yk.y - yp.y);  // 2Y[k] + conj 2Y[k']
+    const float2 g = make_float2(yk.x - yp.x, yk.y + yp.y);  // 2Y[k] - conj 2Y[k']
+    const float2 b = cmulc(g, w);                              // W^-k G
+    // 4V[k] = F + i B, 4V[k'] = conj F + i conj B  (i z = (-z.y, z.x))
+    const float sc = 0.5f / (float)n;
+    zv[k] = make_float2((f.x - b.y) * sc, (f.y + b.x) * sc);
+    if (kp < M && kp != k) zv[kp] = make_float2((f.x + b.y) * sc, (-f.y + b.x) * sc);
+}
+
+hipError_t launch_conv_half_merge(float2 *zv, const float2 *spec, long n, bool corr, hipStream_t st) {
+    if (n < 4 || (n & (n - 1))) return hipErrorInvalidValue;
+    const long threads = (n >> 2) + 1;  // k in [0, n/4]
+    hipLaunchKernelGGL(k_conv_half_merge, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, zv, spec, n,
+                       corr ? 1 : 0);
+    return hipGetLastError();
+}
+
 template <class K>
 static long conv_resident(K kernel, int threads) {
     return co_resident_blocks(reinterpret_cast<const void *>(kernel), threads);

@@ -7,13 +7,18 @@
 //   n >  2^13: the forward global passes (the first reading real pairs), one middle
 //              pass fusing the last forward pass, the spectrum multiply and the first
 //              inverse pass, then the remaining inverse passes (the last scaling and
-//              unpacking): 2P - 1 launches for a P-pass split (pass_tile.h I/O modes).
+//              unpacking): 2P - 1 launches for a P-pass split (pass_tile.h I/O modes);
+//   one FP32 row, n >= 2^14: the row read as n/2 complex values (even and odd samples),
+//              a forward plan of n/2, k_conv_half_merge (the row's spectrum from the
+//              half-length one, the multiply, and back), an inverse plan of n/2 --
+//              half the values of a pair whose second row is empty.
 // ConvBuffers (spectrum, folded filter, two scratch buffers) are separate so a caller
 // can place them: the estl_conv_* object owns its own, the server takes them from its
 // HBM arena.  The filter spectrum is computed once per filter and stays resident.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -37,6 +42,11 @@ hipError_t launch_conv_scales_group(const ConvGroup *g, int members, long max_ro
 hipError_t launch_conv_fp16_filter(const float *h, long k, float2 *spec, long n, const float2 *dc, bool normalized,
                                    hipStream_t st);
 hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1, long rows, hipStream_t st);
+hipError_t launch_conv_half_merge(float2 *zv, const float2 *spec, long n, bool corr, hipStream_t st);
+
+bool conv_half_path(long n, long rows, int precision) {
+    return precision == ESTL_PRECISION_FP32 && rows == 1 && n >= (1L << (kMaxRowsLog2 + 1));
+}
 
 // A filter whose k taps all lie within the first spectrum pass's stride n / B_1,
 // unrotated, puts one nonzero value in each column of that pass, whose DFT is that value
@@ -148,6 +158,14 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precisi
     c->inv.back().args.io_out = 2;
     c->inv.back().args.real_rows = rows;
     c->scratch_bytes = (h ? 4 : sizeof(float2)) * (size_t)n * (size_t)c->pairs;
+    if (conv_half_path(n, rows, precision)) {
+        // the half-length transforms (their scratch, n/2 values, fits the pair's)
+        rc = build_plan(&c->half_fwd, n / 2, 1, ESTL_FORWARD, ESTL_PRECISION_FP32, 0, nullptr);
+        if (rc) return rc;
+        rc = build_plan(&c->half_inv, n / 2, 1, ESTL_BACKWARD, ESTL_PRECISION_FP32, 0, nullptr);
+        if (rc) return rc;
+        c->half = true;
+    }
     return ESTL_OK;
 }
 
@@ -247,10 +265,26 @@ static int conv_pass_chain(const ConvShape &c, const void *x, void *y, const flo
     return ESTL_OK;
 }
 
+// One row through transforms of n/2 values (rows read and written as complex values,
+// so both must be 8-byte aligned): forward plan into scratch0 (scratch1 its
+// workspace), the merge in place, inverse plan into y.  The plans run with caller
+// workspace, which leaves them unchanged.
+static int conv_half_chain(const ConvShape &c, const float2 *spec, const float *x, float *y, float2 *scratch0,
+                           float2 *scratch1, hipStream_t st) {
+    if (!scratch0 || !scratch1) return set_error(ESTL_ERR_INVALID, "the convolution of n=%ld needs scratch", c.n);
+    Plan *fwd = const_cast<Plan *>(&c.half_fwd), *inv = const_cast<Plan *>(&c.half_inv);
+    if (int rc = execute_plan_ws(fwd, reinterpret_cast<const float2 *>(x), scratch0, scratch1, nullptr, st)) return rc;
+    hipError_t e = launch_conv_half_merge(scratch0, spec, c.n, c.corr, st);
+    if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "k_conv_half_merge launch: %s", hipGetErrorString(e));
+    return execute_plan_ws(inv, scratch0, reinterpret_cast<float2 *>(y), scratch1, nullptr, st);
+}
+
 int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y, hipStream_t st) {
     if (!x || !y) return set_error(ESTL_ERR_INVALID, "null buffer");
     if (c.precision != ESTL_PRECISION_FP32)
         return set_error(ESTL_ERR_INVALID, "an FP16-storage convolution runs through estl_conv_execute_fp16");
+    if (c.half && !(reinterpret_cast<uintptr_t>(x) & 7) && !(reinterpret_cast<uintptr_t>(y) & 7))
+        return conv_half_chain(c, b.spec, x, y, b.scratch[0], b.scratch[1], st);
     if (c.fused) {
         hipError_t e = launch_conv_rows(c.log2n, c.corr, x, y, c.rows, b.spec, c.tw_rows, c.scale, st);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "k_conv_rows launch: %s", hipGetErrorString(e));

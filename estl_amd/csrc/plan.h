@@ -102,7 +102,12 @@ struct ConvShape {
     size_t scratch_bytes = 0;        // bytes of each of the two scratch buffers
     size_t spec_bytes = 0;           // bytes of the spectrum (FP16: n + 1 values, the
                                      // last holding the filter's L1 bound)
+    // one FP32 row of n >= 2^14 (conv_half_path): transforms of n/2 values around
+    // k_conv_half_merge, taken when the row's input and output are 8-byte aligned
+    bool half = false;
+    Plan half_fwd, half_inv;
 };
+bool conv_half_path(long n, long rows, int precision);
 struct ConvBuffers {
     float2 *spec = nullptr, *dc = nullptr;
     float2 *scratch[2] = {nullptr, nullptr};

@@ -434,6 +434,9 @@ class Server {
     bool groupable(const Request &r) const {
         if (r.kind != KIND_CONV && r.kind != KIND_CONV_REG) return false;
         if (r.flags & ESTL_CONV_NORMALIZE_KERNEL) return false;
+        // a single FP32 row runs through half-length transforms, which groups do not use
+        const int prec = (r.flags & ESTL_CONV_FP16) ? ESTL_PRECISION_FP16 : ESTL_PRECISION_FP32;
+        if (conv_half_path(r.n, r.rows, prec)) return false;
         if (r.n < 2 || r.rows < 1) return false;
         if (r.kind == KIND_CONV_REG) {
             auto f = reqs_.find(r.filter);

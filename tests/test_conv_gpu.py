@@ -124,6 +124,45 @@ def test_spectrum_first_pass_skip_is_exact(gpu, n, k, period):
         check_rows(y, oracle.conv_circular(x64, h.astype(np.float64)), n)
 
 
+@pytest.mark.parametrize("n,k,flags", [
+    (2 ** 14, 100, 0), (2 ** 14, 2 ** 14 + 5, estl_amd.CONV_CORRELATE), (2 ** 15, 300, estl_amd.CONV_CENTER),
+    (2 ** 16, 1000, estl_amd.CONV_NORMALIZE_KERNEL), (2 ** 18, 64, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER),
+    (2 ** 20, 5000, 0), (2 ** 22, 8192, estl_amd.CONV_CORRELATE),
+])
+def test_single_row_half_length(gpu, n, k, flags):
+    """One FP32 row of n >= 2^14 runs through complex transforms of n/2 values (its even
+    and odd samples as one complex row) with the row's spectrum rebuilt between them.
+    Checked against the convolution theorem in FP64 (every size; also the oracle's
+    direct sum for a plain convolution at 2^14), for convolution, correlation, centring and normalisation; the
+    same row at an odd float offset (not 8-byte aligned) takes the pair path and agrees
+    within the same bound."""
+    import torch
+    x = oracle.fill_uniform(n, 60).astype(np.float32).reshape(1, n)
+    h = (oracle.fill_uniform(k, 61) + (1.25 if flags & estl_amd.CONV_NORMALIZE_KERNEL else 0.0)).astype(np.float32)
+    hp = folded(h.astype(np.float64), n)
+    if flags & estl_amd.CONV_CENTER:
+        hp = np.roll(hp, -(((k + 1) // 2) % n))
+    if flags & estl_amd.CONV_NORMALIZE_KERNEL:
+        hp = hp / hp.sum()
+    H = np.fft.fft(hp)
+    X = np.fft.fft(x.astype(np.float64), axis=1)
+    ref = np.real(np.fft.ifft(X * (np.conj(H) if flags & estl_amd.CONV_CORRELATE else H)[None, :], axis=1))
+    y = conv_gpu(torch, gpu, x, h, flags)
+    check_rows(y, ref, n)
+    if n <= 2 ** 16 and flags == 0:
+        check_rows(y, oracle.conv_circular(x.astype(np.float64), h.astype(np.float64)), n)
+    # misaligned row: the pair path
+    c = estl_amd.Convolution(n, 1, flags)
+    c.set_filter(torch.from_numpy(h).to(gpu))
+    buf = torch.zeros(2 * n + 4, dtype=torch.float32, device=gpu)
+    xin = buf[1:1 + n]
+    xin.copy_(torch.from_numpy(x[0]).to(gpu))
+    yout = buf[n + 3:2 * n + 3]
+    c.execute(xin, yout)
+    torch.cuda.synchronize()
+    check_rows(yout.cpu().numpy().astype(np.float64).reshape(1, n), ref, n)
+
+
 def test_identity_and_delay_closed_forms(gpu):
     import torch
     n, rows = 2 ** 14, 5
