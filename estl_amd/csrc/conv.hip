@@ -274,12 +274,18 @@ hipError_t launch_conv_scales_group(const ConvGroup *g, int members, long max_ro
 // and the inverse DFT_M of V (1/M) is y[2m] + i y[2m+1].  One thread per pair
 // {k, k'}, k in [0, M/2]; at k' the same terms appear conjugated (2 X[k'] = conj E -
 // i conj(W^k D) with E, D the sum and difference above, likewise for V).  V overwrites
-// Z in place, times 1/(2n), so the unnormalised inverse transform yields y.
+// Z in place, times 1/(2n), so the unnormalised inverse transform yields y.  In a
+// request group (workgroup row y = member q) row q of zv is member q's, with its
+// spectrum.
 __global__ void __launch_bounds__(256) k_conv_half_merge(float2 *__restrict__ zv, const float2 *__restrict__ spec,
-                                                         long n, int corr) {
+                                                         long n, int corr, const ConvGroup *__restrict__ group) {
     const long M = n >> 1;
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k > (M >> 1)) return;
+    if (group) {
+        zv += (long)blockIdx.y * M;
+        spec = group->spec[blockIdx.y];
+    }
     const long kp = M - k;  // in [M/2, M]; Z[M] is Z[0]
     const float2 zk = zv[k], zp = zv[kp == M ? 0 : kp];
     double sd, cd;
@@ -303,11 +309,13 @@ __global__ void __launch_bounds__(256) k_conv_half_merge(float2 *__restrict__ zv
     if (kp < M && kp != k) zv[kp] = make_float2((f.x + b.y) * sc, (-f.y + b.x) * sc);
 }
 
-hipError_t launch_conv_half_merge(float2 *zv, const float2 *spec, long n, bool corr, hipStream_t st) {
+hipError_t launch_conv_half_merge(float2 *zv, const float2 *spec, long n, bool corr, hipStream_t st,
+                                  const ConvGroup *group, int members) {
     if (n < 4 || (n & (n - 1))) return hipErrorInvalidValue;
+    if (group ? (members < 1 || members > kGroupMax) : !spec) return hipErrorInvalidValue;
     const long threads = (n >> 2) + 1;  // k in [0, n/4]
-    hipLaunchKernelGGL(k_conv_half_merge, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, zv, spec, n,
-                       corr ? 1 : 0);
+    hipLaunchKernelGGL(k_conv_half_merge, dim3((unsigned)((threads + 255) / 256), group ? (unsigned)members : 1u),
+                       dim3(256), 0, st, zv, spec, n, corr ? 1 : 0, group);
     return hipGetLastError();
 }
 

@@ -42,7 +42,8 @@ hipError_t launch_conv_scales_group(const ConvGroup *g, int members, long max_ro
 hipError_t launch_conv_fp16_filter(const float *h, long k, float2 *spec, long n, const float2 *dc, bool normalized,
                                    hipStream_t st);
 hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1, long rows, hipStream_t st);
-hipError_t launch_conv_half_merge(float2 *zv, const float2 *spec, long n, bool corr, hipStream_t st);
+hipError_t launch_conv_half_merge(float2 *zv, const float2 *spec, long n, bool corr, hipStream_t st,
+                                  const ConvGroup *group = nullptr, int members = 0);
 
 bool conv_half_path(long n, long rows, int precision) {
     return precision == ESTL_PRECISION_FP32 && rows == 1 && n >= (1L << (kMaxRowsLog2 + 1));
@@ -330,6 +331,50 @@ int conv_run_group(const ConvShape &c, const ConvGroup *d_group, long pairs, flo
         return ESTL_OK;
     }
     return conv_pass_chain(c, nullptr, nullptr, nullptr, d_group, pairs, scratch0, scratch1, st);
+}
+
+// A request group of single FP32 rows (conv_half_path, n/2 > 2^13): the forward
+// passes of n/2 over the members' rows (the first reading each member's row as
+// complex values) into z, the merge per member, the inverse passes (the last writing
+// each member's output), ping-ponging with ws.  fwd and inv are plans of n/2 values
+// for any batch: every pass runs over `members` rows.
+int conv_run_group_half(const Plan &fwd, const Plan &inv, const ConvGroup *d_group, int members, long n, bool corr,
+                        float2 *z, float2 *ws, hipStream_t st) {
+    if (fwd.kind != Plan::PASSES || inv.kind != Plan::PASSES || fwd.n * 2 != n || inv.n * 2 != n ||
+        fwd.steps.size() != inv.steps.size() || fwd.steps.size() < 2)
+        return set_error(ESTL_ERR_INVALID, "not a half-length group plan pair for n=%ld", n);
+    if (members < 1 || members > kGroupMax || !z || !ws) return set_error(ESTL_ERR_INVALID, "bad request group");
+    const int P = (int)fwd.steps.size();
+    const float2 *src = nullptr;  // the first pass reads the members' rows
+    for (int q = 0; q < P; ++q) {
+        PassArgs a = fwd.steps[q].args;
+        a.batch = members;
+        if (q == 0) {
+            a.io_in = 5;
+            a.group = d_group;
+        }
+        float2 *dst = ((P - 1 - q) & 1) ? ws : z;
+        hipError_t e = launch_pass(fwd.steps[q].log2b, fwd.steps[q].log2t, ESTL_FORWARD, src, dst, a, st);
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group half-length forward pass: %s", hipGetErrorString(e));
+        src = dst;
+    }
+    hipError_t e = launch_conv_half_merge(z, nullptr, n, corr, st, d_group, members);
+    if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group k_conv_half_merge launch: %s", hipGetErrorString(e));
+    src = z;
+    for (int q = 0; q < P; ++q) {
+        PassArgs a = inv.steps[q].args;
+        a.batch = members;
+        float2 *dst = (q & 1) ? z : ws;
+        if (q == P - 1) {
+            a.io_out = 3;
+            a.group = d_group;
+            dst = nullptr;  // the last pass writes the members' outputs
+        }
+        e = launch_pass(inv.steps[q].log2b, inv.steps[q].log2t, ESTL_BACKWARD, src, dst, a, st);
+        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group half-length inverse pass: %s", hipGetErrorString(e));
+        src = dst;
+    }
+    return ESTL_OK;
 }
 
 // FP16 storage: every member's spectrum divided by its filter's L = sum |h| (kept in

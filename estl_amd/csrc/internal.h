@@ -48,6 +48,8 @@ struct PassArgs {
     //   spectrum / corr: the convolution middle pass (launch_pass_mid) multiplies its
     //   natural-order bins by spectrum[k] (its conjugate if corr)
     //   io_out = 2: write re / im of the scaled result to rout rows 2r, 2r + 1
+    //   io_in  = 5 / io_out = 3 (request groups of single rows): complex row r is
+    //               group member r's input x / output y (each one complex row)
     //   io_in  = 3: the row is a convolution filter built from its fk real taps at rin:
     //               folded modulo n and rotated by fshift (fold_tap); first pass of the
     //               filter's spectrum plan, direct load path (S >= T)

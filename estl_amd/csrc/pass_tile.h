@@ -145,14 +145,20 @@ __device__ __forceinline__ void pass_tile_load(const ST *__restrict__ in, const 
         for (int i = 0; i < E; ++i)
             v[i] = ld_planar(reinterpret_cast<const float2 *>(in),
                              e0 + (ls >= LOG2T ? long((P.tl + TPT * i) << ls) : long(NT * i)));
-    } else if (ls >= LOG2T) {
-        const ST *p = in + (P.row << a.log2n) + ((P.g << (ls + LOG2B)) + P.z);
-#pragma unroll
-        for (int i = 0; i < E; ++i) v[i] = ld_elem(p + ((P.tl + TPT * i) << ls));
     } else {
-        const ST *p = in + (P.row << a.log2n) + (P.j0 << LOG2B);
+        // a plain complex row (io_in 5: a request group member's, one row each)
+        const ST *rowp = in + (P.row << a.log2n);
+        if constexpr (F32)
+            if (a.io_in == 5) rowp = static_cast<const ST *>(a.group->x[P.row]);
+        if (ls >= LOG2T) {
+            const ST *p = rowp + ((P.g << (ls + LOG2B)) + P.z);
 #pragma unroll
-        for (int k = 0; k < E; ++k) v[k] = ld_elem(p + P.t + NT * k);
+            for (int i = 0; i < E; ++i) v[i] = ld_elem(p + ((P.tl + TPT * i) << ls));
+        } else {
+            const ST *p = rowp + (P.j0 << LOG2B);
+#pragma unroll
+            for (int k = 0; k < E; ++k) v[k] = ld_elem(p + P.t + NT * k);
+        }
     }
 }
 
@@ -183,7 +189,7 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
     const int lcpr = a.log2n - LOG2B;
     const int c = P.c, tl = P.tl, j = P.j;
     bool busy = lds_busy;
-    if ((a.io_in == 0 || a.io_in == 4) && ls < LOG2T) {
+    if ((a.io_in == 0 || a.io_in == 4 || a.io_in == 5) && ls < LOG2T) {
         // value o of the contiguous block belongs to column (o / (S B)) S + o mod S,
         // element (o mod S B) / S
         const int sbmask = (1 << (ls + LOG2B)) - 1, smask = (1 << ls) - 1;
@@ -275,6 +281,8 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
         return;
     }
     ST *dst = out + (P.row << a.log2n) + j;
+    if constexpr (F32)
+        if (a.io_out == 3) dst = static_cast<ST *>(a.group->y[P.row]) + j;
 #pragma unroll
     for (int i = 0; i < E; ++i) st_elem(dst + ((tl + TPT * i) << lcpr), cscale(v[i], a.scale));
 }

@@ -108,6 +108,8 @@ struct ConvShape {
     Plan half_fwd, half_inv;
 };
 bool conv_half_path(long n, long rows, int precision);
+int conv_run_group_half(const Plan &fwd, const Plan &inv, const ConvGroup *d_group, int members, long n, bool corr,
+                        float2 *z, float2 *ws, hipStream_t st);
 struct ConvBuffers {
     float2 *spec = nullptr, *dc = nullptr;
     float2 *scratch[2] = {nullptr, nullptr};

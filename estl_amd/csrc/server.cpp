@@ -149,6 +149,7 @@ struct Request {
     ConvShape *gshape = nullptr;       // leader: the group shape of (n, flags, pair bucket)
     long gpairs = 0;                   // leader: the group's pairs
     Plan *gspec = nullptr;             // leader: spectrum plan over the group's filters
+    Plan *gfwd = nullptr, *ginv = nullptr;  // leader of single rows: half-length plans
     int gslot = -1;                    // leader: its pinned descriptor slot
 };
 
@@ -419,6 +420,18 @@ class Server {
         return c.spec_plan.kind == Plan::PASSES ? c.spec_plan.scratch_bytes : 0;
     }
 
+    Plan *half_plan(long m, long batch, int dir) {
+        auto key = std::make_tuple(m, batch, dir);
+        auto it = half_plans_.find(key);
+        if (it != half_plans_.end()) return it->second.get();
+        std::unique_ptr<Plan> p(new Plan());
+        if (build_plan(p.get(), m, batch, dir, ESTL_PRECISION_FP32, 0, nullptr)) return nullptr;
+        if (p->kind != Plan::PASSES) return nullptr;
+        Plan *raw = p.get();
+        half_plans_[key] = std::move(p);
+        return raw;
+    }
+
     Plan *group_spec_plan(long n, long filters) {
         auto key = std::make_pair(n, filters);
         auto it = group_spec_plans_.find(key);
@@ -430,13 +443,19 @@ class Server {
         return raw;
     }
 
-    // a convolution whose filter is ready (inline, or registered and done)
+    // a single FP32 row of n >= 2^14, which runs through half-length transforms
+    static bool half_row(const Request &r) {
+        const int prec = (r.flags & ESTL_CONV_FP16) ? ESTL_PRECISION_FP16 : ESTL_PRECISION_FP32;
+        return conv_half_path(r.n, r.rows, prec);
+    }
+
+    // a convolution whose filter is ready (inline, or registered and done); single rows
+    // on the half-length path group with one another when their passes are global
+    // (n/2 > 2^13) and their input can be read as complex values, as it is alone
     bool groupable(const Request &r) const {
         if (r.kind != KIND_CONV && r.kind != KIND_CONV_REG) return false;
         if (r.flags & ESTL_CONV_NORMALIZE_KERNEL) return false;
-        // a single FP32 row runs through half-length transforms, which groups do not use
-        const int prec = (r.flags & ESTL_CONV_FP16) ? ESTL_PRECISION_FP16 : ESTL_PRECISION_FP32;
-        if (conv_half_path(r.n, r.rows, prec)) return false;
+        if (half_row(r) && (r.n <= (2L << kMaxRowsLog2) || (reinterpret_cast<uintptr_t>(r.in) & 7))) return false;
         if (r.n < 2 || r.rows < 1) return false;
         if (r.kind == KIND_CONV_REG) {
             auto f = reqs_.find(r.filter);
@@ -457,7 +476,8 @@ class Server {
         const size_t window = std::min(queue_.size(), (size_t)kGroupWindow);
         for (size_t i = 1; i < window && (int)m.size() < kGroupMax; ++i) {
             Request &o = reqs_.at(queue_[i]);
-            if (o.kind != r.kind || o.n != r.n || o.flags != r.flags || !groupable(o)) continue;
+            if (o.kind != r.kind || o.n != r.n || o.flags != r.flags || half_row(o) != half_row(r) || !groupable(o))
+                continue;
             if (values + o.n * o.rows > kGroupMaxValues) continue;
             m.push_back(&o);
             pairs += (o.rows + 1) / 2;
@@ -476,6 +496,13 @@ class Server {
         // multi-pass sizes: the spectra through a spectrum plan over the G filters
         Plan *gsp = (inline_filters && !gs->fused) ? group_spec_plan(r.n, G) : nullptr;
         if (inline_filters && !gs->fused && !gsp) return false;
+        // single rows: half-length plans for the bucket's batch (launched with G rows)
+        Plan *gfwd = nullptr, *ginv = nullptr;
+        if (half_row(r)) {
+            gfwd = half_plan(r.n / 2, 1L << bucket, ESTL_FORWARD);
+            ginv = half_plan(r.n / 2, 1L << bucket, ESTL_BACKWARD);
+            if (!gfwd || !ginv) return false;
+        }
         // blocks: every member's output, then the leader's workspace (multi-pass: two
         // scratch buffers; inline filters: the spectra; the descriptor)
         const bool fp16 = (r.flags & ESTL_CONV_FP16) != 0;
@@ -525,6 +552,8 @@ class Server {
         for (size_t i = m.size(); i < blocks.size(); ++i) r.blocks.push_back(blocks[i]);
         r.gshape = gs;
         r.gspec = gsp;
+        r.gfwd = gfwd;
+        r.ginv = ginv;
         r.gpairs = pairs;
         r.gslot = free_slots_.back();
         free_slots_.pop_back();
@@ -598,6 +627,9 @@ class Server {
             for (int q = 0; q < g.count; ++q) skip = skip && spectrum_skips_first_pass(*r.gspec, g.k[q], g.shift[q]);
             if (int rc = conv_filter_group(*r.gspec, d_desc, spectra, scr0, skip, st)) return rc;
         }
+        if (r.gfwd)
+            return conv_run_group_half(*r.gfwd, *r.ginv, d_desc, g.count, n, (r.flags & ESTL_CONV_CORRELATE) != 0, scr0,
+                                       scr1, st);
         if (fp16) {
             if (inline_filters)
                 if (int rc = conv_fp16_filter_group(d_desc, g.count, n, st)) return rc;
@@ -940,6 +972,7 @@ class Server {
     Clock::time_point first_submit_, last_done_;
     // request groups
     std::map<std::pair<long, long>, std::unique_ptr<Plan>> group_spec_plans_;  // (n, filters)
+    std::map<std::tuple<long, long, int>, std::unique_ptr<Plan>> half_plans_;  // (n / 2, batch, direction)
     ConvGroup *host_desc_ = nullptr;  // kGroupSlots pinned descriptors
     std::vector<int> free_slots_;
     long long groups_ = 0, grouped_ = 0;

@@ -410,6 +410,53 @@ def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags, registered):
     srv.close()
 
 
+@pytest.mark.parametrize("n,flags", [(2 ** 16, 0), (2 ** 20, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER)])
+def test_single_row_groups_equal_alone_bitwise(gpu, n, flags):
+    """Single FP32 rows run through transforms of n/2 values; queued single rows of one
+    length group with one another (each member's row read and written as one complex
+    row of the group's passes, a spectrum merge per member), never with multi-row
+    requests of the same length, and every result stays bit-identical to the request
+    alone, with inline and then registered filters."""
+    import torch
+    rows_list = [1, 1, 3, 1, 1, 2, 1, 1, 1, 4, 1]
+    ks = [16, 300, 100, 4097, 64, 2048, 1, 999, 8192, 33, 700]
+    cases = [("conv", n, r, k, flags) for r, k in zip(rows_list, ks)]
+    inputs = [make_inputs(torch, gpu, c, 900 + i) for i, c in enumerate(cases)]
+    refs = [alone(torch, gpu, c, x, h) for c, (x, h) in zip(cases, inputs)]
+    srv = estl_amd.Server(4 << 30, streams=1)
+    side = _hold(torch)
+    ids = []
+    for i, c in enumerate(cases):
+        kind, n_, rows, k, fl = c
+        x, h = inputs[i]
+        ids.append((i, srv.submit_conv(n_, rows, x, h, k, fl, stream=side if i < 2 else None)))
+    srv.drain(120)
+    for i, rid in ids:
+        srv.wait(rid, 60)
+        assert torch.equal(result(srv, rid, cases[i], torch), refs[i]), (i, rid)
+        srv.release(rid)
+    st = srv.stats()
+    assert st["failed"] == 0 and st["groups"] >= 2 and st["grouped_requests"] >= 6, st
+    fids = [srv.register_filter(n, h, int(h.numel()), flags & ~estl_amd.CONV_CORRELATE) for _, h in inputs]
+    for f in fids:
+        srv.wait(f, 60)
+    srv.reset_stats()
+    side = _hold(torch)
+    ids = [(i, srv.submit_conv_filter(fids[i], cases[i][2], inputs[i][0], flags & estl_amd.CONV_CORRELATE,
+                                      stream=side if i < 2 else None)) for i in range(len(cases))]
+    srv.drain(120)
+    for i, rid in ids:
+        srv.wait(rid, 60)
+        assert torch.equal(result(srv, rid, cases[i], torch), refs[i]), (i, rid)
+        srv.release(rid)
+    st = srv.stats()
+    assert st["failed"] == 0 and st["groups"] >= 2 and st["grouped_requests"] >= 6, st
+    for f in fids:
+        srv.release(f)
+    assert srv.stats()["arena_used_bytes"] == 0
+    srv.close()
+
+
 def test_request_groups_at_member_limit(gpu):
     """Forty queued requests of one length form several groups of at most 16 members
     (the descriptor's limit); every result stays bit-identical to the request alone."""
