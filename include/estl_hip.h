@@ -216,7 +216,9 @@ int estl_ubatch_destroy(estl_ubatch_t ub);
  * two) and a real filter of any length k (longer than n: folds modulo n), computed as
  * transform, multiply, inverse -- the pipeline of Convolution2D_FFT::convolve
  * (Convolution.h:95-148), batched over 1-D rows and fused on the device.  Two real rows
- * share one complex transform.  Flags: ESTL_CONV_* above. */
+ * share one complex transform; a single FP32 row of n >= 2^14 whose input and output
+ * are 8-byte aligned runs through complex transforms of n/2 values instead (its even
+ * and odd samples as one complex row).  Flags: ESTL_CONV_* above. */
 int estl_conv_create(estl_conv_t *conv, long n, long rows, int precision, unsigned flags);
 
 /* Fold (and, with ESTL_CONV_CENTER, rotate) the k-value device filter d_h and keep its
@@ -256,7 +258,9 @@ int estl_conv_destroy(estl_conv_t conv);
  * is DONE.  Request groups: a convolution (no kernel normalisation; a registered
  * filter must be ready) admitted from the head of the queue takes along queued
  * requests of the same kind, n and flags from the next 256 queue entries (at most 16
- * requests, 2^24 values), which are thereby admitted ahead of requests between them; the group runs as one batch and its members complete
+ * requests, 2^24 values; single FP32 rows, which run through transforms of n/2
+ * values, only with single rows), which are thereby admitted ahead of requests
+ * between them; the group runs as one batch and its members complete
  * together.  Each member keeps its id, output and release.  Results are bit-identical
  * to the same request served alone, grouped or not. */
 #define ESTL_REQ_QUEUED 0
