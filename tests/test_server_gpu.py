@@ -410,13 +410,16 @@ def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags, registered):
     srv.close()
 
 
-@pytest.mark.parametrize("n,flags", [(2 ** 16, 0), (2 ** 20, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER)])
-def test_single_row_groups_equal_alone_bitwise(gpu, n, flags):
+@pytest.mark.parametrize("n,flags,min_grouped", [(2 ** 16, 0, 6),
+                                                 (2 ** 20, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER, 6),
+                                                 (2 ** 14, estl_amd.CONV_CORRELATE, 2)])
+def test_single_row_groups_equal_alone_bitwise(gpu, n, flags, min_grouped):
     """Single FP32 rows run through transforms of n/2 values; queued single rows of one
     length group with one another (each member's row read and written as one complex
     row of the group's passes, a spectrum merge per member), never with multi-row
     requests of the same length, and every result stays bit-identical to the request
-    alone, with inline and then registered filters."""
+    alone, with inline and then registered filters.  At n = 2^14 (transforms of 2^13
+    in one launch each) single rows run alone and only the multi-row requests group."""
     import torch
     rows_list = [1, 1, 3, 1, 1, 2, 1, 1, 1, 4, 1]
     ks = [16, 300, 100, 4097, 64, 2048, 1, 999, 8192, 33, 700]
@@ -436,7 +439,7 @@ def test_single_row_groups_equal_alone_bitwise(gpu, n, flags):
         assert torch.equal(result(srv, rid, cases[i], torch), refs[i]), (i, rid)
         srv.release(rid)
     st = srv.stats()
-    assert st["failed"] == 0 and st["groups"] >= 2 and st["grouped_requests"] >= 6, st
+    assert st["failed"] == 0 and st["groups"] >= 1 and st["grouped_requests"] >= min_grouped, st
     fids = [srv.register_filter(n, h, int(h.numel()), flags & ~estl_amd.CONV_CORRELATE) for _, h in inputs]
     for f in fids:
         srv.wait(f, 60)
@@ -450,7 +453,7 @@ def test_single_row_groups_equal_alone_bitwise(gpu, n, flags):
         assert torch.equal(result(srv, rid, cases[i], torch), refs[i]), (i, rid)
         srv.release(rid)
     st = srv.stats()
-    assert st["failed"] == 0 and st["groups"] >= 2 and st["grouped_requests"] >= 6, st
+    assert st["failed"] == 0 and st["groups"] >= 1 and st["grouped_requests"] >= min_grouped, st
     for f in fids:
         srv.release(f)
     assert srv.stats()["arena_used_bytes"] == 0
