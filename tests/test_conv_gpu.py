@@ -163,6 +163,42 @@ def test_single_row_half_length(gpu, n, k, flags):
     check_rows(yout.cpu().numpy().astype(np.float64).reshape(1, n), ref, n)
 
 
+def test_random_configurations_match_fp64(gpu):
+    """Forty seeded configurations over the whole shape space -- n = 2^4..2^22, 1..9 rows
+    (capped at 2^22 values), filters of 1..2n taps, every flag combination -- each
+    checked row by row against the convolution theorem in FP64 (numpy's FFT of the
+    folded, rotated, normalised filter).  Covers every pipeline: the fused rows kernel,
+    the pass chain with its middle pass, single rows through half-length transforms,
+    and the spectrum plan with and without its first pass."""
+    import torch
+    rng = np.random.default_rng(2024)
+    for case in range(40):
+        L = int(rng.integers(4, 23))
+        n = 1 << L
+        rows = int(min(rng.integers(1, 10), max(1, (1 << 22) // n)))
+        k = int(rng.integers(1, 2 * n + 1)) if rng.random() < 0.3 else int(rng.integers(1, min(n, 8192) + 1))
+        bits = int(rng.integers(0, 8))
+        flags = ((estl_amd.CONV_CORRELATE if bits & 1 else 0) | (estl_amd.CONV_NORMALIZE_KERNEL if bits & 2 else 0) |
+                 (estl_amd.CONV_CENTER if bits & 4 else 0))
+        x = oracle.fill_uniform(rows * n, 100 + case).astype(np.float32).reshape(rows, n)
+        h = oracle.fill_uniform(k, 200 + case).astype(np.float32)
+        if flags & estl_amd.CONV_NORMALIZE_KERNEL:
+            h = (h + 1.5).astype(np.float32)
+        hp = folded(h.astype(np.float64), n)
+        if flags & estl_amd.CONV_CENTER:
+            hp = np.roll(hp, -(((k + 1) // 2) % n))
+        if flags & estl_amd.CONV_NORMALIZE_KERNEL:
+            hp = hp / hp.sum()
+        H = np.fft.fft(hp)
+        if flags & estl_amd.CONV_CORRELATE:
+            H = np.conj(H)
+        ref = np.real(np.fft.ifft(np.fft.fft(x.astype(np.float64), axis=1) * H[None, :], axis=1))
+        y = conv_gpu(torch, gpu, x, h, flags)
+        for r in range(rows):
+            e = oracle.rel_l2(y[r], ref[r])
+            assert e <= tol(n), (case, n, rows, k, flags, r, e)
+
+
 def test_identity_and_delay_closed_forms(gpu):
     import torch
     n, rows = 2 ** 14, 5
