@@ -49,11 +49,12 @@ __device__ __forceinline__ int conv_group_member(const ConvGroup *g, long pair) 
 // The spectrum of a filter of length n <= 2^13 in one workgroup: fold the taps on
 // load, forward FFT (the rows kernel's block_fft and table, so the bins are those of the
 // rows plan), optionally divide by the kernel sum Re K^[0] (Convolution.h:141), store.
-// With a request group, workgroup q builds member q's filter into its spectrum.
+// With a request group, workgroup q builds member q's filter into its spectrum, divided
+// by the member's L (group->l1[q].x) for FP16 storage when divide_l1 is set.
 template <int LOG2N>
 __global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT)
 k_filter_spectrum(const float *__restrict__ h, long k, long shift, float2 *__restrict__ spec,
-                  const float2 *__restrict__ tw, int normalize, const ConvGroup *__restrict__ group) {
+                  const float2 *__restrict__ tw, int normalize, const ConvGroup *__restrict__ group, int divide_l1) {
     using G = BlockGeom<LOG2N>;
     constexpr int N = G::N, E = G::E, TPT = G::TPT;
     __shared__ float2 lds[G::uses_lds ? Lds<kLdsPad>::size(N) : 1];
@@ -77,19 +78,25 @@ k_filter_spectrum(const float *__restrict__ h, long k, long shift, float2 *__res
 #pragma unroll
         for (int i = 0; i < E; ++i) v[i] = make_float2(v[i].x * s, v[i].y * s);
     }
+    if (group && divide_l1) {
+        const float s = 1.0f / group->l1[blockIdx.x].x;
+#pragma unroll
+        for (int i = 0; i < E; ++i) v[i] = make_float2(v[i].x * s, v[i].y * s);
+    }
 #pragma unroll
     for (int i = 0; i < E; ++i) spec[tl + TPT * i] = v[i];
 }
 
 hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift, float2 *spec, const float2 *tw,
-                                  bool normalize, hipStream_t st, const ConvGroup *group, int members) {
+                                  bool normalize, hipStream_t st, const ConvGroup *group, int members,
+                                  bool divide_l1) {
     if (group && (members < 1 || members > kGroupMax || normalize)) return hipErrorInvalidValue;
     const unsigned blocks = group ? (unsigned)members : 1u;
     switch (log2n) {
 #define FS(L)                                                                                                   \
     case L:                                                                                                     \
         hipLaunchKernelGGL(k_filter_spectrum<L>, dim3(blocks), dim3(BlockGeom<L>::TPT), 0, st, h, k, shift, spec, \
-                           tw, normalize ? 1 : 0, group);                                                        \
+                           tw, normalize ? 1 : 0, group, divide_l1 ? 1 : 0);                                     \
         break;
         FS(1) FS(2) FS(3) FS(4) FS(5) FS(6) FS(7) FS(8) FS(9) FS(10) FS(11) FS(12) FS(13)
 #undef FS
@@ -215,31 +222,19 @@ hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1,
     return hipGetLastError();
 }
 
-// The same three steps for every member of a request group (workgroup row y = member),
-// in the same order and arithmetic as for one request, the (sum |h|, L) pair of member q
-// kept in g->l1[q].
+// The same steps for every member of a request group, with the single request's
+// arithmetic: member q's (sum |h|, L) pair in g->l1[q] (workgroup q), written before
+// the spectra, whose last launch stores them divided by L (1.0f / L, the factor
+// k_spec_l1 applies); then the output row scales (workgroup row y = member).
 __global__ void __launch_bounds__(256) k_filter_l1_group(ConvGroup *__restrict__ g) {
     const int q = blockIdx.x;
     __shared__ float part[256];
     const float v = filter_l1(g->h[q], g->k[q], part);
-    if (threadIdx.x == 0) g->l1[q].y = v;
-}
-
-__global__ void k_spec_l1_group(ConvGroup *__restrict__ g, long n) {
-    const int q = blockIdx.y;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    float L = g->l1[q].y;
-    if (!(L > 0.0f)) L = 1.0f;
-    if (i < n) {
-        float2 *spec = const_cast<float2 *>(g->spec[q]);
-        const float s = 1.0f / L;
-        spec[i] = make_float2(spec[i].x * s, spec[i].y * s);
-    }
-    if (i == 0) g->l1[q].x = L;
+    if (threadIdx.x == 0) g->l1[q] = make_float2(v > 0.0f ? v : 1.0f, v);
 }
 
 // output row scales of every member, s_out = s_in * L: L from the descriptor (inline
-// filters, written by k_spec_l1_group) or from the value after the member's n spectrum
+// filters, written by k_filter_l1_group) or from the value after the member's n spectrum
 // values (registered filters, written by k_spec_l1 when the filter was registered)
 __global__ void k_conv_scales_group(const ConvGroup *__restrict__ g, long n, int registered) {
     const int q = blockIdx.y;
@@ -250,10 +245,9 @@ __global__ void k_conv_scales_group(const ConvGroup *__restrict__ g, long n, int
     }
 }
 
-hipError_t launch_conv_fp16_filter_group(ConvGroup *g, int members, long n, hipStream_t st) {
+hipError_t launch_conv_fp16_filter_group(ConvGroup *g, int members, hipStream_t st) {
     if (members < 1 || members > kGroupMax) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_filter_l1_group, dim3((unsigned)members), dim3(256), 0, st, g);
-    hipLaunchKernelGGL(k_spec_l1_group, dim3((unsigned)((n + 255) / 256), (unsigned)members), dim3(256), 0, st, g, n);
     return hipGetLastError();
 }
 

@@ -32,11 +32,12 @@ namespace estl {
 hipError_t launch_conv_rows(int log2n, bool corr, const float *x, float *y, long rows, const float2 *spec,
                             const float2 *tw, float scale, hipStream_t st, const ConvGroup *group = nullptr);
 hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift, float2 *spec, const float2 *tw,
-                                  bool normalize, hipStream_t st, const ConvGroup *group = nullptr, int members = 0);
+                                  bool normalize, hipStream_t st, const ConvGroup *group = nullptr, int members = 0,
+                                  bool divide_l1 = false);
 hipError_t launch_spec_normalize(float2 *spec, long n, const float2 *dc, hipStream_t st);
 hipError_t launch_conv_rows_fp16(int log2n, bool corr, const void *x, void *y, long rows, const float2 *spec,
                                  const float2 *tw, float scale, hipStream_t st, const ConvGroup *group = nullptr);
-hipError_t launch_conv_fp16_filter_group(ConvGroup *g, int members, long n, hipStream_t st);
+hipError_t launch_conv_fp16_filter_group(ConvGroup *g, int members, hipStream_t st);
 hipError_t launch_conv_scales_group(const ConvGroup *g, int members, long max_rows, long n, bool registered,
                                     hipStream_t st);
 hipError_t launch_conv_fp16_filter(const float *h, long k, float2 *spec, long n, const float2 *dc, bool normalized,
@@ -60,8 +61,8 @@ bool spectrum_skips_first_pass(const Plan &spec, long k, long shift) {
 // The passes of a spectrum plan over the filter(s) given by `a0` (taps, count, rotation,
 // or a group descriptor): the first reads the taps (io_in = 3), or, when `skip`, the
 // second does (io_in = 4); destinations alternate back from `spec` with `scratch0`.
-static int spectrum_passes(const Plan &plan, const PassArgs &taps, bool skip, float2 *spec, float2 *scratch0,
-                           hipStream_t st) {
+static int spectrum_passes(const Plan &plan, const PassArgs &taps, bool skip, bool divide_l1, float2 *spec,
+                           float2 *scratch0, hipStream_t st) {
     const auto &steps = plan.steps;
     const int P = (int)steps.size();
     const int q0 = skip ? 1 : 0;
@@ -75,6 +76,7 @@ static int spectrum_passes(const Plan &plan, const PassArgs &taps, bool skip, fl
             a.fshift = taps.fshift;
         }
         a.group = taps.group;
+        if (divide_l1 && q == P - 1) a.io_out = 4;
         float2 *dst = ((P - 1 - q) & 1) ? scratch0 : spec;
         hipError_t e = launch_pass(steps[q].log2b, steps[q].log2t, ESTL_FORWARD, src, dst, a, st);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "filter spectrum pass: %s", hipGetErrorString(e));
@@ -197,8 +199,8 @@ int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k
         taps.rin = h;
         taps.fk = k;
         taps.fshift = shift;
-        if (int rc = spectrum_passes(c.spec_plan, taps, spectrum_skips_first_pass(c.spec_plan, k, shift), b.spec,
-                                     b.scratch[0], st))
+        if (int rc = spectrum_passes(c.spec_plan, taps, spectrum_skips_first_pass(c.spec_plan, k, shift), false,
+                                     b.spec, b.scratch[0], st))
             return rc;
     }
     if (normalize && c.spec_plan.kind != Plan::ROWS) {
@@ -297,22 +299,23 @@ int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y,
 // A request group (ConvGroup in device memory at d_group).  Spectra: the G filters as
 // the G rows of `spec` (a spectrum plan over G rows whose first pass builds them from
 // their taps), destinations alternating back from `spectra` with scratch0.
-int conv_filter_group_rows(const ConvShape &c, const ConvGroup *d_group, int members, hipStream_t st) {
+int conv_filter_group_rows(const ConvShape &c, const ConvGroup *d_group, int members, bool divide_l1,
+                           hipStream_t st) {
     if (!c.fused || c.spec_plan.kind != Plan::ROWS) return set_error(ESTL_ERR_INVALID, "not a fused shape");
     hipError_t e = launch_filter_spectrum(c.log2n, nullptr, 0, 0, nullptr, c.spec_plan.tw_rows, false, st, d_group,
-                                          members);
+                                          members, divide_l1);
     if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group filter spectra: %s", hipGetErrorString(e));
     return ESTL_OK;
 }
 
 int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectra, float2 *scratch0,
-                      bool skip_first, hipStream_t st) {
+                      bool skip_first, bool divide_l1, hipStream_t st) {
     if (spec.kind != Plan::PASSES || spec.steps.empty() || spec.steps.front().args.io_in != 3)
         return set_error(ESTL_ERR_INVALID, "not a group spectrum plan");
     if (skip_first && spec.steps.size() < 2) return set_error(ESTL_ERR_INVALID, "no first pass to skip");
     PassArgs taps{};
     taps.group = d_group;
-    return spectrum_passes(spec, taps, skip_first, spectra, scratch0, st);
+    return spectrum_passes(spec, taps, skip_first, divide_l1, spectra, scratch0, st);
 }
 
 // The convolution passes of a request group over its `pairs` pairs (the members' pairs
@@ -377,12 +380,13 @@ int conv_run_group_half(const Plan &fwd, const Plan &inv, const ConvGroup *d_gro
     return ESTL_OK;
 }
 
-// FP16 storage: every member's spectrum divided by its filter's L = sum |h| (kept in
-// the descriptor; a registered filter's spectrum was divided when it was registered,
-// with L after its n values), the passes on binary16 rows, then the members' output
-// row scales.
-int conv_fp16_filter_group(ConvGroup *d_group, int members, long n, hipStream_t st) {
-    hipError_t e = launch_conv_fp16_filter_group(d_group, members, n, st);
+// FP16 storage: every member's spectrum divided by its filter's L = sum |h|.  Inline
+// filters: L into the descriptor before the spectra (conv_fp16_filter_group), whose
+// last launch stores them divided by it (divide_l1); a registered filter's spectrum
+// was divided when it was registered, with L after its n values.  Then the passes on
+// binary16 rows and the members' output row scales (conv_run_group_fp16).
+int conv_fp16_filter_group(ConvGroup *d_group, int members, hipStream_t st) {
+    hipError_t e = launch_conv_fp16_filter_group(d_group, members, st);
     if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group FP16 filter scaling: %s", hipGetErrorString(e));
     return ESTL_OK;
 }

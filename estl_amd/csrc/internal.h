@@ -50,6 +50,8 @@ struct PassArgs {
     //   io_out = 2: write re / im of the scaled result to rout rows 2r, 2r + 1
     //   io_in  = 5 / io_out = 3 (request groups of single rows): complex row r is
     //               group member r's input x / output y (each one complex row)
+    //   io_out = 4: the last pass of a request group's FP16 spectra: row r stored times
+    //               1 / group->l1[r].x (member r's L), in place of the scale
     //   io_in  = 3: the row is a convolution filter built from its fk real taps at rin:
     //               folded modulo n and rotated by fshift (fold_tap); first pass of the
     //               filter's spectrum plan, direct load path (S >= T)

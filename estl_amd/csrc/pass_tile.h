@@ -281,10 +281,13 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
         return;
     }
     ST *dst = out + (P.row << a.log2n) + j;
-    if constexpr (F32)
+    float sc = a.scale;
+    if constexpr (F32) {
         if (a.io_out == 3) dst = static_cast<ST *>(a.group->y[P.row]) + j;
+        if (a.io_out == 4) sc = 1.0f / a.group->l1[P.row].x;
+    }
 #pragma unroll
-    for (int i = 0; i < E; ++i) st_elem(dst + ((tl + TPT * i) << lcpr), cscale(v[i], a.scale));
+    for (int i = 0; i < E; ++i) st_elem(dst + ((tl + TPT * i) << lcpr), cscale(v[i], sc));
 }
 
 // Computes tile `tile` of the pass described by `a` (load, then finish).

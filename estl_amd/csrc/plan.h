@@ -125,10 +125,13 @@ int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k
 int group_spectrum_plan(Plan *p, long n, long filters);
 // skip_first: every member's filter passes spectrum_skips_first_pass (its k, its shift)
 bool spectrum_skips_first_pass(const Plan &spec, long k, long shift);
+// divide_l1 (FP16 storage): the spectra stored divided by the members' L, which
+// conv_fp16_filter_group wrote into the descriptor beforehand
 int conv_filter_group(const Plan &spec, const ConvGroup *d_group, float2 *spectra, float2 *scratch0,
-                      bool skip_first, hipStream_t st);
-int conv_filter_group_rows(const ConvShape &c, const ConvGroup *d_group, int members, hipStream_t st);
-int conv_fp16_filter_group(ConvGroup *d_group, int members, long n, hipStream_t st);
+                      bool skip_first, bool divide_l1, hipStream_t st);
+int conv_filter_group_rows(const ConvShape &c, const ConvGroup *d_group, int members, bool divide_l1,
+                           hipStream_t st);
+int conv_fp16_filter_group(ConvGroup *d_group, int members, hipStream_t st);
 // registered: the members' spectra are registered filters' (already divided by L, which
 // follows their n values)
 int conv_run_group_fp16(const ConvShape &c, ConvGroup *d_group, int members, long pairs, long max_rows,

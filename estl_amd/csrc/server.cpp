@@ -619,22 +619,23 @@ class Server {
         if (pair != r.gpairs) return set_error(ESTL_ERR_INVALID, "group pairs %ld != admitted pairs %ld", pair, r.gpairs);
         if (hipMemcpyAsync(d_desc, &g, sizeof g, hipMemcpyHostToDevice, st) != hipSuccess)
             return set_error(ESTL_ERR_HIP, "group descriptor upload failed");
+        // FP16 storage with inline filters: the members' L first, the spectra are stored
+        // divided by it
+        const bool divide_l1 = fp16 && inline_filters;
+        if (divide_l1)
+            if (int rc = conv_fp16_filter_group(d_desc, g.count, st)) return rc;
         if (inline_filters && fused) {
-            if (int rc = conv_filter_group_rows(*r.gshape, d_desc, g.count, st)) return rc;
+            if (int rc = conv_filter_group_rows(*r.gshape, d_desc, g.count, divide_l1, st)) return rc;
         } else if (inline_filters) {
             if (r.gspec->batch != g.count) return set_error(ESTL_ERR_INVALID, "group spectrum plan size");
             bool skip = true;  // the first spectrum pass is skipped when every filter allows it
             for (int q = 0; q < g.count; ++q) skip = skip && spectrum_skips_first_pass(*r.gspec, g.k[q], g.shift[q]);
-            if (int rc = conv_filter_group(*r.gspec, d_desc, spectra, scr0, skip, st)) return rc;
+            if (int rc = conv_filter_group(*r.gspec, d_desc, spectra, scr0, skip, divide_l1, st)) return rc;
         }
         if (r.gfwd)
             return conv_run_group_half(*r.gfwd, *r.ginv, d_desc, g.count, n, (r.flags & ESTL_CONV_CORRELATE) != 0, scr0,
                                        scr1, st);
-        if (fp16) {
-            if (inline_filters)
-                if (int rc = conv_fp16_filter_group(d_desc, g.count, n, st)) return rc;
-            return conv_run_group_fp16(*r.gshape, d_desc, g.count, pair, max_rows, !inline_filters, scr0, scr1, st);
-        }
+        if (fp16) return conv_run_group_fp16(*r.gshape, d_desc, g.count, pair, max_rows, !inline_filters, scr0, scr1, st);
         return conv_run_group(*r.gshape, d_desc, pair, scr0, scr1, st);
     }
 

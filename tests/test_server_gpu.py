@@ -200,9 +200,12 @@ def test_registered_filters_and_completion_queue(gpu):
         refs.append(alone(torch, gpu, ("conv", n, rows, k, fl), x, h))
         fids.append(srv.register_filter(n, h, k))  # not waited on: FIFO order covers it
     ids = {}
+    # the requests wait on a busy input stream, so they are still in flight when the
+    # filter's release is attempted below
+    side = _hold(torch)
     for rep in range(3):
         for i, (n, rows, k, fl) in enumerate(cases):
-            ids[srv.submit_conv_filter(fids[i], rows, inputs[i][0], fl)] = i
+            ids[srv.submit_conv_filter(fids[i], rows, inputs[i][0], fl, stream=side)] = i
     # a filter in use cannot be released
     with pytest.raises(estl_amd.EstlError) as e:
         srv.release(fids[0])
