@@ -237,6 +237,55 @@ def bench_large(torch, estl_amd, stream, reps: int = 10, check: bool = True):
     return out
 
 
+def bench_fft2d(torch, estl_amd, stream, reps: int = 10):
+    """2-D transform on the device (FFT2D's rows then columns, estl_plan2d_*): one
+    4096 x 4096 complex64 array (128 MiB), uniform[-1,1) seed 5, forward, median of
+    `reps` launches pairs timed with HIP events on the stream.  Accuracy: a unit 2-D
+    tone transforms to exactly nx*ny at its bin (rel-L2 over the whole spectrum from
+    Parseval) and forward + normalised backward returns the input."""
+    nx = ny = 4096
+    n = nx * ny
+    x = torch.empty(ny, nx, dtype=torch.complex64, device="cuda:0")
+    estl_amd.fill_uniform(x, 5, stream=stream)
+    y = torch.empty_like(x)
+    fwd = estl_amd.Plan2D(nx, ny)
+    bwd = estl_amd.Plan2D(nx, ny, direction=estl_amd.BACKWARD, flags=estl_amd.FLAG_NORMALIZE)
+    for _ in range(3):
+        fwd.execute(x, y, stream)
+    ev = []
+    for _ in range(reps):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        fwd.execute(x, y, stream)
+        e.record(stream)
+        ev.append((s, e))
+    stream.synchronize()
+    ms = sorted(s.elapsed_time(e) for s, e in ev)[reps // 2]
+    z = torch.empty_like(x)
+    bwd.execute(y, z, stream)
+    stream.synchronize()
+    rt = (torch.linalg.vector_norm((z - x).to(torch.complex128)) /
+          torch.linalg.vector_norm(x.to(torch.complex128))).item()
+    fx, fy = 1234, 567
+    j = torch.arange(ny, device="cuda:0", dtype=torch.float64).reshape(ny, 1)
+    i = torch.arange(nx, device="cuda:0", dtype=torch.float64).reshape(1, nx)
+    ph = 2 * math.pi * (torch.remainder(i * fx, nx) / nx + torch.remainder(j * fy, ny) / ny)
+    t = torch.polar(torch.ones_like(ph), ph).to(torch.complex64)
+    del ph
+    torch.cuda.synchronize()  # t was made on torch's stream
+    fwd.execute(t, y, stream)
+    stream.synchronize()
+    peak = complex(y[fy, fx].item())
+    energy = (y.abs().double() ** 2).sum().item()
+    tone = math.sqrt(max(energy - abs(peak) ** 2, 0.0) + abs(peak - n) ** 2) / n
+    return {"workload": "2-D C2C fp32 forward, 4096 x 4096, uniform[-1,1) seed 5, HBM-resident",
+            "ms": round(ms, 4), "launches": fwd.launches(),
+            "gbs": round(16.0 * n / (ms * 1e-3) / 1e9, 1),
+            "hbm_traffic_gbs": round(16.0 * n * fwd.launches() / (ms * 1e-3) / 1e9, 1),
+            "gflops_5nlog2n": round(5.0 * n * math.log2(n) / (ms * 1e-3) / 1e9, 1),
+            "tone_rel_l2": tone, "round_trip_rel_l2": rt, "bound": 1e-6 * math.log2(n)}
+
+
 def _drive(srv, reqs, xp, hp, arrivals, filters=None, keep=(), fp16=None):
     """Submit request i at time arrivals[i] (s from start) -- with its own filter
     (hp[:k], spectrum computed inside the request) or, when `filters` maps (n, bank
@@ -456,6 +505,7 @@ def main() -> int:
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-size batch leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served convolution stream leg")
     ap.add_argument("--no-large", action="store_true", help="skip the N = 2^26 (FP32 and FP16 storage) leg")
+    ap.add_argument("--no-2d", action="store_true", help="skip the 4096 x 4096 2-D transform leg")
     ap.add_argument("--percall-rows", type=int, default=BATCH)
     ap.add_argument("--graph", action="store_true",
                     help="replay the step's two launches as one HIP graph (measured: no gain over eager launches)")
@@ -616,6 +666,7 @@ def main() -> int:
     uneven = None if args.no_uneven else bench_uneven(torch, estl_amd, stream)
     serve = None if args.no_serve else bench_serve(torch, estl_amd, stream)
     large = None if args.no_large else bench_large(torch, estl_amd, stream)
+    fft2d = None if args.no_2d else bench_fft2d(torch, estl_amd, stream)
 
     cpu = None if args.no_cpu else cpu_baseline(BATCH, 3)
     cpu_lib = None if args.no_cpu else cpu_library(BATCH)
@@ -655,6 +706,7 @@ def main() -> int:
         "uneven_batch": uneven,
         "served_convolution": serve,
         "large_n": large,
+        "fft2d": fft2d,
         "cpu_baseline": cpu,
         "cpu_library_all_cores": cpu_lib,
     }

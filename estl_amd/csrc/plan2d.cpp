@@ -63,8 +63,18 @@ static int build_plan2d(Plan2D *p, long nx, long ny, long batch, int dir, unsign
         if (rc) return rc;
     }
     if (p->has_cols) {
+        // Up to ny = 2^9 the columns take one pass of radix ny, whose tiles keep 2^(13 -
+        // log2 ny) >= 16 adjacent columns (128-byte runs).  A longer column keeps fewer
+        // (two at 4096: 16-byte runs at a 32 KiB stride), so ny = 2^10..2^13 takes two
+        // passes of radix <= 2^7 with 64-column tiles (512-byte runs): one more HBM round
+        // trip, measured much faster.  Above 2^13 the default split applies.
         std::vector<int> split;
-        if (ly <= kMaxRowsLog2) split.push_back(ly);  // one pass of radix ny
+        if (ly <= 9 || (ly <= kMaxRowsLog2 && nx < 4)) {
+            split.push_back(ly);
+        } else if (ly <= kMaxRowsLog2) {
+            split.push_back(ly / 2);
+            split.push_back(ly - ly / 2);
+        }
         int rc = build_plan(&p->cols, ny, 1, dir, ESTL_PRECISION_FP32, 0, split.empty() ? nullptr : &split);
         if (rc) return rc;
         // rescale the ny plan's passes to columns of stride nx

@@ -3,8 +3,8 @@
 // Same public surface as the reference's FFT2D (FFT.h:54-84): `data` (xDim * yDim
 // Complex<D>, x fastest), xDim, yDim, forward and a synchronous execute().  Where FFT2D
 // runs one 1-D execute -- one host round trip -- per row and then per column, execute()
-// here is one estl_plan2d_execute_host call: copy in, a batched row launch and a strided
-// column pass on the device, copy out.  Complex<float> is two floats (re, im), so `data`
+// here is one estl_plan2d_execute_host call: copy in, a batched row launch and strided
+// column passes on the device, copy out.  Complex<float> is two floats (re, im), so `data`
 // is exactly interleaved complex64.
 #ifndef ESTL_HIP_FFT2D_HIP_H
 #define ESTL_HIP_FFT2D_HIP_H

@@ -115,8 +115,9 @@ int estl_execute_device(estl_plan_t plan, const void *d_in, void *d_out, void *s
  * `batch` arrays of ny rows of nx complex64, x fastest (FFT2D's data[y * xDim + x],
  * FFT.h:54-84, which runs one host round trip per row and per column).  Rows are one
  * batched 1-D launch; columns are Stockham passes at stride nx (no transpose; one pass
- * for ny <= 2^13), so the transform makes two HBM round trips.  ESTL_FLAG_NORMALIZE
- * scales by 1/(nx ny).  d_in == d_out is allowed. */
+ * for ny <= 2^9, two for ny up to 2^13, so that every tile moves runs of at least 16
+ * adjacent columns), so the transform makes two or three HBM round trips.
+ * ESTL_FLAG_NORMALIZE scales by 1/(nx ny).  d_in == d_out is allowed. */
 int estl_plan2d_create(estl_plan2d_t *plan, long nx, long ny, long batch, int direction, unsigned flags);
 int estl_plan2d_execute(estl_plan2d_t plan, const void *d_in, void *d_out, void *stream);
 /* Synchronous host-buffer form (copy in, transform, copy out), like FFT2D::execute. */

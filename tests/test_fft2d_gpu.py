@@ -49,7 +49,8 @@ def test_plan2d_matches_oracle(gpu, nx, ny, batch):
         e = oracle.rel_l2(got[b], ref[b])
         assert e <= tol(nx, ny), (nx, ny, b, e)
     if ny > 1 and ny <= 8192 and nx > 1 and nx <= 8192:
-        assert p.launches() == 2          # one row launch, one column pass
+        # one row launch; one column pass up to ny = 2^9 (or nx < 4), two above
+        assert p.launches() == (2 if ny <= 512 or nx < 4 else 3), p.launches()
 
 
 def test_plan2d_bruteforce_tiny(gpu):
