@@ -352,19 +352,23 @@ def test_request_groups_equal_alone_bitwise(gpu, n, flags):
     srv.close()
 
 
-@pytest.mark.parametrize("n,flags,registered", [
-    (2 ** 16, estl_amd.CONV_CORRELATE, False), (2 ** 11, estl_amd.CONV_CENTER, False),
-    (2 ** 16, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER, True), (2 ** 12, estl_amd.CONV_CENTER, True),
+@pytest.mark.parametrize("n,flags,registered,short", [
+    (2 ** 16, estl_amd.CONV_CORRELATE, False, False), (2 ** 11, estl_amd.CONV_CENTER, False, False),
+    (2 ** 16, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER, True, False), (2 ** 12, estl_amd.CONV_CENTER, True, False),
+    (2 ** 16, 0, False, True), (2 ** 19, estl_amd.CONV_CORRELATE, False, True),
 ])
-def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags, registered):
+def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags, registered, short):
     """FP16-storage requests of one length, queued behind a busy input stream, run as a
-    group: with inline filters the filters' L1 norms are kept in the group's descriptor,
-    with registered filters (centring a filter option, correlation a request option) L
-    follows each filter's spectrum.  Each binary16 output and its row scales are
-    bit-identical to the FP16 convolution object's."""
+    group: with inline filters the filters' L1 norms are kept in the group's descriptor
+    and the spectra are stored divided by them, with registered filters (centring a
+    filter option, correlation a request option) L follows each filter's spectrum.  With
+    `short` filters every tap lies within the first spectrum pass's stride, so the
+    group's spectra skip that pass (at 2^16 the one pass left both reads the taps and
+    divides by L).  Each binary16 output and its row scales are bit-identical to the
+    FP16 convolution object's."""
     import torch
     rows_list = [3, 1, 8, 5, 2, 7]
-    ks = [100, 16, 2000, 33, 8192, 1]
+    ks = [100, 16, 200, 33, 256, 1] if short else [100, 16, 2000, 33, 8192, 1]
     refs, keep = [], []
     for i, (rows, k) in enumerate(zip(rows_list, ks)):
         x = torch.empty(rows * n, dtype=torch.float32, device=gpu)
