@@ -50,11 +50,14 @@ __device__ __forceinline__ int conv_group_member(const ConvGroup *g, long pair) 
 // load, forward FFT (the rows kernel's block_fft and table, so the bins are those of the
 // rows plan), optionally divide by the kernel sum Re K^[0] (Convolution.h:141), store.
 // With a request group, workgroup q builds member q's filter into its spectrum, divided
-// by the member's L (group->l1[q].x) for FP16 storage when divide_l1 is set.
+// by the member's L (group->l1[q].x) for FP16 storage when divide_l1 is set.  With
+// normalisation the kernel sum K^[0] is also written to dc (the FP16 L1 bound divides
+// by it).
 template <int LOG2N>
 __global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT)
 k_filter_spectrum(const float *__restrict__ h, long k, long shift, float2 *__restrict__ spec,
-                  const float2 *__restrict__ tw, int normalize, const ConvGroup *__restrict__ group, int divide_l1) {
+                  const float2 *__restrict__ tw, int normalize, const ConvGroup *__restrict__ group, int divide_l1,
+                  float2 *__restrict__ dc) {
     using G = BlockGeom<LOG2N>;
     constexpr int N = G::N, E = G::E, TPT = G::TPT;
     __shared__ float2 lds[G::uses_lds ? Lds<kLdsPad>::size(N) : 1];
@@ -72,7 +75,10 @@ k_filter_spectrum(const float *__restrict__ h, long k, long shift, float2 *__res
     for (int i = 0; i < E; ++i) v[i] = make_float2(fold_tap(h, k, N, shift, tl + TPT * i), 0.0f);
     block_fft<LOG2N, -1, kLdsPad>(v, lds, tl, tw, false);
     if (normalize) {
-        if (tl == 0) s_dc = v[0].x;  // bin 0
+        if (tl == 0) {
+            s_dc = v[0].x;  // bin 0
+            if (dc) *dc = v[0];
+        }
         __syncthreads();
         const float s = 1.0f / s_dc;
 #pragma unroll
@@ -89,14 +95,14 @@ k_filter_spectrum(const float *__restrict__ h, long k, long shift, float2 *__res
 
 hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift, float2 *spec, const float2 *tw,
                                   bool normalize, hipStream_t st, const ConvGroup *group, int members,
-                                  bool divide_l1) {
+                                  bool divide_l1, float2 *dc) {
     if (group && (members < 1 || members > kGroupMax || normalize)) return hipErrorInvalidValue;
     const unsigned blocks = group ? (unsigned)members : 1u;
     switch (log2n) {
 #define FS(L)                                                                                                   \
     case L:                                                                                                     \
         hipLaunchKernelGGL(k_filter_spectrum<L>, dim3(blocks), dim3(BlockGeom<L>::TPT), 0, st, h, k, shift, spec, \
-                           tw, normalize ? 1 : 0, group, divide_l1 ? 1 : 0);                                     \
+                           tw, normalize ? 1 : 0, group, divide_l1 ? 1 : 0, dc);                                 \
         break;
         FS(1) FS(2) FS(3) FS(4) FS(5) FS(6) FS(7) FS(8) FS(9) FS(10) FS(11) FS(12) FS(13)
 #undef FS

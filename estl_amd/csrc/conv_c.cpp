@@ -33,7 +33,7 @@ hipError_t launch_conv_rows(int log2n, bool corr, const float *x, float *y, long
                             const float2 *tw, float scale, hipStream_t st, const ConvGroup *group = nullptr);
 hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift, float2 *spec, const float2 *tw,
                                   bool normalize, hipStream_t st, const ConvGroup *group = nullptr, int members = 0,
-                                  bool divide_l1 = false);
+                                  bool divide_l1 = false, float2 *dc = nullptr);
 hipError_t launch_spec_normalize(float2 *spec, long n, const float2 *dc, hipStream_t st);
 hipError_t launch_conv_rows_fp16(int log2n, bool corr, const void *x, void *y, long rows, const float2 *spec,
                                  const float2 *tw, float scale, hipStream_t st, const ConvGroup *group = nullptr);
@@ -189,8 +189,9 @@ int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k
     const bool normalize = (c.flags & ESTL_CONV_NORMALIZE_KERNEL) != 0;
     hipError_t e = hipSuccess;
     if (c.spec_plan.kind == Plan::ROWS) {
-        // fold, transform and normalise in one launch
-        e = launch_filter_spectrum(c.log2n, h, k, shift, b.spec, c.spec_plan.tw_rows, normalize, st);
+        // fold, transform and normalise in one launch (the kernel sum to b.dc)
+        e = launch_filter_spectrum(c.log2n, h, k, shift, b.spec, c.spec_plan.tw_rows, normalize, st, nullptr, 0,
+                                   false, b.dc);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "filter spectrum launch: %s", hipGetErrorString(e));
     } else {
         // the spectrum plan's passes, the first run building the filter from its taps

@@ -319,3 +319,32 @@ def test_conv_fp16_output_scale_is_input_scale_times_l1(gpu):
     np.testing.assert_allclose(s_out.cpu().numpy(), s_in.cpu().numpy() * l1, rtol=1e-6)
     with pytest.raises(estl_amd.EstlError):
         c.execute(dx, dx)            # an FP16 object has no FP32 entry point
+
+
+@pytest.mark.parametrize("n", [4096, 2 ** 14])
+def test_conv_fp16_normalised_output_scale(gpu, n):
+    """With kernel normalisation the filter is h / sum(h), whose L1 bound is
+    sum|h| / |sum h|: the output scale is the input scale times that (the spectrum of
+    n <= 2^13 is normalised inside its one launch, which also records the kernel sum;
+    larger n normalise after the spectrum passes)."""
+    import torch
+    rows, k = 3, 500
+    x = oracle.fill_uniform(rows * n, 42).astype(np.float32).reshape(rows, n)
+    h = (oracle.fill_uniform(k, 43) + 0.5).astype(np.float32)     # mixed signs, nonzero sum
+    c = estl_amd.Convolution(n, rows, estl_amd.CONV_NORMALIZE_KERNEL, precision=estl_amd.FP16)
+    dx = torch.from_numpy(x).to(gpu)
+    x16 = torch.empty(rows * n, dtype=torch.float16, device=gpu)
+    s_in = torch.empty(rows, dtype=torch.float32, device=gpu)
+    estl_amd.pack_fp16(dx, x16, s_in, n // 2, rows)
+    c.set_filter(torch.from_numpy(h).to(gpu))
+    y16, s_out = torch.empty_like(x16), torch.empty_like(s_in)
+    c.execute_fp16(x16, s_in, y16, s_out)
+    torch.cuda.synchronize()
+    h64 = h.astype(np.float64)
+    L = np.abs(h64).sum() / abs(h64.sum())
+    np.testing.assert_allclose(s_out.cpu().numpy(), s_in.cpu().numpy() * L, rtol=1e-5)
+    y = (y16.float().view(rows, n) * s_out[:, None]).cpu().numpy().astype(np.float64)
+    xq = (x16.float().view(rows, n) * s_in[:, None]).cpu().numpy().astype(np.float64)
+    ref = oracle.conv_circular(xq, h64 / h64.sum())
+    for r in range(rows):
+        assert oracle.rel_l2(y[r], ref[r]) <= fp16_tol(n), r

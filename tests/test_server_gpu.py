@@ -467,6 +467,91 @@ def test_single_row_groups_equal_alone_bitwise(gpu, n, flags, min_grouped):
     srv.close()
 
 
+def test_random_stream_equals_alone_bitwise(gpu):
+    """Sixty seeded requests over the whole served space -- n = 2^10..2^20, 1..6 rows,
+    filters of 1..3000 taps, every flag, FP32 and FP16 storage, inline and registered
+    filters -- submitted at once (the first two held on a busy input stream, so the rest
+    queue, group and run beside one another on two streams).  Every output (and FP16
+    row scales) is bit-identical to the same convolution run alone through the object
+    API, whatever group, path (pairs, single rows at half length, fused rows) or
+    neighbours it had."""
+    import torch
+    rng = np.random.default_rng(99)
+    flag_bits = (estl_amd.CONV_CORRELATE, estl_amd.CONV_CENTER, estl_amd.CONV_NORMALIZE_KERNEL)
+    cases = []
+    for i in range(60):
+        n = 1 << int(rng.integers(10, 21))
+        rows = int(rng.integers(1, 7))
+        k = int(rng.integers(1, 3001))
+        flags = sum(f for f in flag_bits if rng.random() < 0.3)
+        fp16 = bool(rng.random() < 0.3)
+        reg = bool(rng.random() < 0.4)
+        cases.append((n, rows, k, flags, fp16, reg))
+    srv = estl_amd.Server(4 << 30, streams=2)
+    data, refs, fids = [], [], {}
+    for i, (n, rows, k, flags, fp16, reg) in enumerate(cases):
+        x = torch.empty(rows * n, dtype=torch.float32, device=gpu)
+        estl_amd.fill_uniform(x, 1000 + i)
+        h = torch.empty(k, dtype=torch.float32, device=gpu)
+        estl_amd.fill_uniform(h, 2000 + i)
+        if flags & estl_amd.CONV_NORMALIZE_KERNEL:
+            h += 1.0
+        c = estl_amd.Convolution(n, rows, flags, precision=estl_amd.FP16 if fp16 else estl_amd.FP32)
+        c.set_filter(h)
+        if fp16:
+            x16 = torch.empty(rows * n, dtype=torch.float16, device=gpu)
+            s_in = torch.empty(rows, dtype=torch.float32, device=gpu)
+            estl_amd.pack_fp16(x, x16, s_in, n // 2, rows)
+            y16, s_out = torch.empty_like(x16), torch.empty_like(s_in)
+            c.execute_fp16(x16, s_in, y16, s_out)
+            data.append((x16, s_in, h))
+            refs.append((y16, s_out))
+        else:
+            y = torch.empty_like(x)
+            c.execute(x, y)
+            data.append((x, None, h))
+            refs.append((y, None))
+        if reg:
+            ff = (flags & ~estl_amd.CONV_CORRELATE) | (estl_amd.CONV_FP16 if fp16 else 0)
+            fids[i] = srv.register_filter(n, h, k, ff)
+    torch.cuda.synchronize()
+    for f in fids.values():
+        srv.wait(f, 60)
+    side = _hold(torch)
+    ids = []
+    for i, (n, rows, k, flags, fp16, reg) in enumerate(cases):
+        x, s_in, h = data[i]
+        st = side if i < 2 else None
+        if reg and fp16:
+            rid = srv.submit_conv_filter_fp16(fids[i], rows, x, s_in, flags & estl_amd.CONV_CORRELATE, stream=st)
+        elif reg:
+            rid = srv.submit_conv_filter(fids[i], rows, x, flags & estl_amd.CONV_CORRELATE, stream=st)
+        elif fp16:
+            rid = srv.submit_conv_fp16(n, rows, x, s_in, h, k, flags, stream=st)
+        else:
+            rid = srv.submit_conv(n, rows, x, h, k, flags, stream=st)
+        ids.append(rid)
+    srv.drain(120)
+    for i, rid in enumerate(ids):
+        n, rows, k, flags, fp16, reg = cases[i]
+        srv.wait(rid, 60)
+        if fp16:
+            y16, s_out = srv.output_fp16(rid, n, rows)
+            torch.cuda.synchronize()
+            assert torch.equal(y16, refs[i][0]) and torch.equal(s_out, refs[i][1]), (i, cases[i])
+        else:
+            y = srv.output(rid, (rows * n,), torch.float32)
+            torch.cuda.synchronize()
+            assert torch.equal(y, refs[i][0]), (i, cases[i])
+        srv.release(rid)
+    st = srv.stats()
+    assert st["failed"] == 0 and st["completed"] == len(ids) and st["groups"] >= 1, st
+    for f in fids.values():
+        srv.release(f)
+    assert srv.stats()["arena_used_bytes"] == 0
+    srv.close()
+
+
 def test_request_groups_at_member_limit(gpu):
     """Forty queued requests of one length form several groups of at most 16 members
     (the descriptor's limit); every result stays bit-identical to the request alone."""
