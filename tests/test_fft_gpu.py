@@ -94,6 +94,37 @@ def test_pass_plans_parity(gpu, n, split, batch):
         assert max_row_err(y, oracle.fft64(xh, direction)) <= tol(n), (n, split, direction)
 
 
+def test_random_plans_match_fp64(gpu):
+    """Forty seeded plans over the plan space: n = 2^1..2^20, 1..5 rows, a random split
+    into 1..4 passes of radix 2^1..2^13 (or the default plan), random tile widths
+    within the limits, either direction, out of place or in place -- each row against
+    the FP64 oracle."""
+    import torch
+    rng = np.random.default_rng(4242)
+    for case in range(40):
+        L = int(rng.integers(1, 21))
+        n = 1 << L
+        batch = int(rng.integers(1, 6)) if L <= 16 else 1
+        split, tiles = None, None
+        if L >= 2 and rng.random() < 0.7:
+            P = int(rng.integers(2, min(4, L) + 1))
+            while True:
+                cuts = sorted(rng.choice(np.arange(1, L), size=P - 1, replace=False).tolist())
+                split = [b - a for a, b in zip([0] + cuts, cuts + [L])]
+                if max(split) <= 13:
+                    break
+            if rng.random() < 0.5:
+                tiles = [int(rng.integers(0, max(0, min(6, 14 - b, L - b)) + 1)) for b in split]
+        direction = estl_amd.FORWARD if rng.random() < 0.5 else estl_amd.BACKWARD
+        in_place = bool(rng.random() < 0.3)
+        x = dev_uniform(torch, batch, n, seed=500 + case, device=gpu)
+        xh = x.cpu().numpy()
+        p = estl_amd.Plan(n, batch, direction, split=split, tiles=tiles)
+        y = run(torch, p, x, out=x if in_place else None)
+        e = max_row_err(y.cpu().numpy(), oracle.fft64(xh, direction))
+        assert e <= tol(n), (case, n, batch, split, tiles, direction, in_place, e)
+
+
 @pytest.mark.parametrize("n,split", [(4096, None), (2 ** 16, None), (2 ** 16, [4, 4, 4, 4]), (2 ** 20, None)])
 def test_in_place(gpu, n, split):
     import torch
