@@ -53,6 +53,38 @@ def test_plan2d_matches_oracle(gpu, nx, ny, batch):
         assert p.launches() == (2 if ny <= 512 or nx < 4 else 3), p.launches()
 
 
+def test_plan2d_random_shapes(gpu):
+    """Twenty seeded 2-D shapes (nx, ny = 2^U{0..14}, at most 2^22 values, 1..3 arrays),
+    either direction, normalised or not, in place or not, against the oracle's
+    separable FP64 transform."""
+    import torch
+    rng = np.random.default_rng(77)
+    for case in range(20):
+        while True:
+            lx, ly = int(rng.integers(0, 15)), int(rng.integers(0, 15))
+            if lx + ly <= 22:
+                break
+        nx, ny = 1 << lx, 1 << ly
+        batch = int(rng.integers(1, 4)) if lx + ly <= 20 else 1
+        direction = estl_amd.FORWARD if rng.random() < 0.5 else estl_amd.BACKWARD
+        norm = bool(rng.random() < 0.3)
+        in_place = bool(rng.random() < 0.3)
+        x = torch.empty(batch, ny, nx, dtype=torch.complex64, device=gpu)
+        estl_amd.fill_uniform(x, 300 + case)
+        xh = x.cpu().numpy().astype(np.complex128)
+        p = estl_amd.Plan2D(nx, ny, batch, direction, estl_amd.FLAG_NORMALIZE if norm else 0)
+        y = x if in_place else torch.empty_like(x)
+        p.execute(x, y)
+        torch.cuda.synchronize()
+        ref = ref2d(xh, direction)
+        if norm:
+            ref = ref / (nx * ny)
+        got = y.cpu().numpy()
+        for b in range(batch):
+            e = oracle.rel_l2(got[b], ref[b])
+            assert e <= tol(nx, ny), (case, nx, ny, batch, direction, norm, in_place, e)
+
+
 def test_plan2d_bruteforce_tiny(gpu):
     import torch
     nx, ny = 8, 4

@@ -49,6 +49,41 @@ def test_every_size_class_against_oracle(gpu):
             off += n
 
 
+def test_random_batches_against_oracle(gpu):
+    """Eight seeded batches: 5..60 transforms of sizes 2^U{1..18} each, placed at random
+    (sorted, non-overlapping) offsets with gaps, random direction, eager or graph
+    replay -- every transform against the FP64 oracle, the gaps untouched."""
+    import torch
+    rng = np.random.default_rng(321)
+    for case in range(8):
+        count = int(rng.integers(5, 61))
+        sizes = [1 << int(k) for k in rng.integers(1, 19, size=count)]
+        offsets, pos = [], 0
+        for n in sizes:
+            pos += int(rng.integers(0, 100))
+            offsets.append(pos)
+            pos += n
+        order = rng.permutation(count)          # offsets in any order of the size list
+        sizes = [sizes[i] for i in order]
+        offsets = [offsets[i] for i in order]
+        extent = pos
+        direction = estl_amd.FORWARD if rng.random() < 0.5 else estl_amd.BACKWARD
+        graph = bool(rng.random() < 0.5)
+        x = torch.empty(extent, dtype=torch.complex64, device=gpu)
+        estl_amd.fill_uniform(x, 900 + case)
+        y = torch.zeros_like(x)
+        ub = estl_amd.UnevenBatch(sizes, offsets=offsets, direction=direction)
+        ub.execute(x, y, graph=graph)
+        torch.cuda.synchronize()
+        xh, yh = x.cpu().numpy(), y.cpu().numpy()
+        mask = np.ones(extent, bool)
+        for o, n in zip(offsets, sizes):
+            e = oracle.rel_l2(yh[o:o + n], oracle.fft64(xh[o:o + n], direction))
+            assert e <= tol(n), (case, n, o, direction, graph, e)
+            mask[o:o + n] = False
+        assert not np.any(yh[mask]), case
+
+
 def test_custom_offsets_and_in_place(gpu):
     import torch
     sizes = [256, 65536, 16, 8192, 4096]
