@@ -73,6 +73,37 @@ def test_planar_equals_interleaved(gpu, n, batch, direction):
         assert np.all(got[~used] == marker)
 
 
+def test_planar_random_pass_plans(gpu):
+    """Twelve seeded pass plans (random split and tile widths, so the planar first-pass
+    loads and last-pass stores take both their direct and their staged paths) give
+    bit-identical values to the interleaved plan of the same split and tiles."""
+    import torch
+    rng = np.random.default_rng(17)
+    for case in range(12):
+        L = int(rng.integers(6, 19))
+        n = 1 << L
+        batch = int(rng.integers(1, 4))
+        P = int(rng.integers(2, 4))
+        while True:
+            cuts = sorted(rng.choice(np.arange(1, L), size=P - 1, replace=False).tolist())
+            split = [b - a for a, b in zip([0] + cuts, cuts + [L])]
+            if max(split) <= 13:
+                break
+        tiles = [int(rng.integers(0, max(0, min(6, 14 - b, L - b)) + 1)) for b in split]
+        direction = estl_amd.FORWARD if rng.random() < 0.5 else estl_amd.BACKWARD
+        count = n * batch
+        z = oracle.uniform_complex(batch, n, 40 + case).astype(np.complex64).reshape(-1)
+        ref_in = torch.from_numpy(z).to(gpu)
+        ref_out = torch.empty_like(ref_in)
+        estl_amd.Plan(n, batch, direction, split=split, tiles=tiles).execute(ref_in, ref_out)
+        pin = torch.from_numpy(to_planar(z)).to(gpu)
+        pout = torch.zeros_like(pin)
+        estl_amd.Plan(n, batch, direction, flags=estl_amd.FLAG_PLANAR, split=split, tiles=tiles).execute(pin, pout)
+        torch.cuda.synchronize()
+        got = from_planar(pout.cpu().numpy(), count)
+        assert np.array_equal(got, ref_out.cpu().numpy()), (case, n, batch, split, tiles, direction)
+
+
 def test_planar_in_place_and_host(gpu):
     import torch
     n, batch = 2 ** 15, 2
