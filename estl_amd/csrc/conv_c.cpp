@@ -46,6 +46,7 @@ hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1,
 hipError_t launch_conv_half_merge(float2 *zv, const float2 *spec, long n, bool corr, hipStream_t st,
                                   const ConvGroup *group = nullptr, int members = 0);
 
+// a single FP32 row long enough for global passes (n >= 2^14) runs at half length
 bool conv_half_path(long n, long rows, int precision) {
     return precision == ESTL_PRECISION_FP32 && rows == 1 && n >= (1L << (kMaxRowsLog2 + 1));
 }
@@ -58,9 +59,11 @@ bool spectrum_skips_first_pass(const Plan &spec, long k, long shift) {
     return k <= (spec.n >> spec.steps.front().log2b);
 }
 
-// The passes of a spectrum plan over the filter(s) given by `a0` (taps, count, rotation,
-// or a group descriptor): the first reads the taps (io_in = 3), or, when `skip`, the
-// second does (io_in = 4); destinations alternate back from `spec` with `scratch0`.
+// The passes of a spectrum plan over the filter(s) given by `taps` (rin, fk, fshift, or a
+// group descriptor): the first reads the taps (io_in = 3), or, when `skip`, the second
+// does (io_in = 4); with `divide_l1` the last stores each group member's spectrum
+// divided by its L (io_out = 4).  Destinations alternate back from `spec` with
+// `scratch0`.
 static int spectrum_passes(const Plan &plan, const PassArgs &taps, bool skip, bool divide_l1, float2 *spec,
                            float2 *scratch0, hipStream_t st) {
     const auto &steps = plan.steps;
