@@ -12,9 +12,10 @@
 //   * continuous batching: admitted requests are launched at once on a pool of streams
 //     and run concurrently with the ones already on the device; their memory is
 //     returned the moment they complete, admitting the next;
-//   * request groups: an FP32 convolution at the head of the queue takes along the
-//     queued requests of the same kind, length and flags found in the next
-//     kGroupWindow entries (up to kGroupMax requests, kGroupMaxValues values), and the
+//   * request groups: a convolution (FP32 or FP16 storage) at the head of the queue
+//     takes along the queued requests of the same kind, length and flags found in the
+//     next kGroupWindow entries (up to kGroupMax requests, kGroupMaxValues values;
+//     single FP32 rows, which run at half length, only with single rows), and the
 //     group runs as one batch: one launch per pass for all of them (the pass kernels
 //     find each row's request in a ConvGroup descriptor) instead of one chain of
 //     launches per request.  Each member keeps its own output, id and release;
