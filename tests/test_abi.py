@@ -72,3 +72,14 @@ def test_server_rejects_bad_arguments_without_a_device():
     assert L.estl_server_wait(None, 1, 0.0) == estl_amd.ERR_INVALID
     assert L.estl_server_release(None, 1) == estl_amd.ERR_INVALID
     assert L.estl_server_destroy(None) == estl_amd.OK
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with a message saying so, before
+    any device work."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode != 0
+    assert "one GPU only" in (r.stdout + r.stderr)
