@@ -12,26 +12,41 @@
 // (the reference over-allocates through a rounding slip, Complex.h:108-110), and the
 // Q14 fixed-point short specialisations the reference marks as not working
 // (Complex.h:132-152) are not provided.
+//
+// Sizes: element indices stay `int` as in the reference, so an array holds at most
+// INT_MAX + 1 = 2^31 scalars.  Element counts are computed in 64 bits and a size that
+// does not fit (or is negative) throws std::length_error; the reference computes them
+// in `int` (Complex.h:23,106,160) and overflows past 2^30 complex values.
 #ifndef ESTL_HIP_COMPLEX_H
 #define ESTL_HIP_COMPLEX_H
 
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <stdexcept>
 #include <string>
 
 #define GlobalAlignLevel 5
 #define GlobalPlannarLevel 5
+
+// most scalars an array can index with an int; the most complex values an interleaved
+// or split ComplexArray holds (its last scalar sits at index 2^31 - 1)
+static const long long kEstlMaxScalars = (long long)INT_MAX + 1;
+static const long long kEstlMaxComplexValues = kEstlMaxScalars / 2;
 
 template <class D>
 class AlignedArray {
   public:
     D *alignedData;
 
-    explicit AlignedArray(int size, int alignLevel = GlobalAlignLevel) {
+    // `size` scalars (an int argument converts; 64-bit so that a count of 2^31 is exact)
+    explicit AlignedArray(long long size, int alignLevel = GlobalAlignLevel) {
+        if (size < 0 || size > kEstlMaxScalars)
+            throw std::length_error("AlignedArray: " + std::to_string(size) + " scalars cannot be indexed by int");
         const size_t align = size_t(1) << alignLevel;
-        raw_ = new unsigned char[size_t(size > 0 ? size : 1) * sizeof(D) + align - 1];
+        raw_ = new unsigned char[size_t(size) * sizeof(D) + align - 1];
         const uintptr_t p = reinterpret_cast<uintptr_t>(raw_);
         alignedData = reinterpret_cast<D *>((p + align - 1) & ~(uintptr_t)(align - 1));
     }
@@ -129,9 +144,15 @@ inline bool operator==(const Complex<R> a, const Complex<R> b) {
 template <int plannarLevel, class D>
 class PlannarizedComplexArray {
   public:
-    explicit PlannarizedComplexArray(int size)
-        : size_(size), elements_(((size + (1 << plannarLevel) - 1) >> plannarLevel) << (plannarLevel + 1)) {
-        data_ = new AlignedArray<D>(elements_ > 0 ? elements_ : 1);
+    explicit PlannarizedComplexArray(int size) : size_(size), elements_(0) {
+        const long long tiles = ((long long)(size > 0 ? size : 0) + (1 << plannarLevel) - 1) >> plannarLevel;
+        const long long elements = tiles << (plannarLevel + 1);
+        // getElements() returns an int, so the tiles must stay below 2^31 scalars
+        if (size < 0 || elements > INT_MAX)
+            throw std::length_error("PlannarizedComplexArray: " + std::to_string(size) +
+                                    " values need more scalars than an int holds");
+        elements_ = int(elements);
+        data_ = new AlignedArray<D>(elements_);
         map_ = new Map(*data_);
     }
     ~PlannarizedComplexArray() {
@@ -162,7 +183,9 @@ class ComplexArray {
   public:
     // planar = true: all reals then all imaginaries; false: interleaved (re, im) pairs
     explicit ComplexArray(int size, bool planar = true) : size_(size), planar_(planar) {
-        data_ = new AlignedArray<D>(2 * (size > 0 ? size : 1));
+        if (size < 0 || size > kEstlMaxComplexValues)
+            throw std::length_error("ComplexArray: " + std::to_string(size) + " complex values exceed 2^30");
+        data_ = new AlignedArray<D>(2LL * size);
     }
     ~ComplexArray() { delete data_; }
     ComplexArray(const ComplexArray &) = delete;

@@ -10,7 +10,9 @@
 //
 // Semantics are FFTW's (FFT_FFTW3.h:26-28): forward = exp(-2 pi i t k / n), backward =
 // exp(+...), both unnormalised, applied to every batch row.  Setup failures (size not
-// a power of two, no device, out of memory) throw std::runtime_error; a failure inside
+// a power of two, no device, out of memory) throw std::runtime_error, and a size whose
+// host arrays an int cannot index (n * batchCount of 2^30 complex values or more, or
+// non-positive) throws std::length_error before anything is allocated; a failure inside
 // execute() is printed and leaves the output untouched, the reference's behaviour
 // (FFT_OpenCL_Contiguous.h:120-122).
 #ifndef ESTL_HIP_FFT_HIP_H
@@ -27,6 +29,36 @@
 #include <FFT.h>
 #include <estl_hip.h>
 
+// largest batch of length-n rows the device's free memory holds (64-bit query; the
+// reference's int query overflows on large devices, uOpenCL.h:323-337), capped so that
+// n * batch host values stay within `max_values` (what the host arrays can index);
+// budget_bytes = 0 asks the device, a positive value stands for its free memory
+inline int estl_dropin_max_batch(int n, long long max_values, long long budget_bytes = 0) {
+    if (n < 1) return 1;
+    long long b = estl_max_batch(n, ESTL_PRECISION_FP32, budget_bytes);
+    const long long cap = max_values / n;
+    if (b > cap) b = cap;
+    if (b > INT_MAX) b = INT_MAX;
+    return b < 1 ? 1 : int(b);
+}
+
+// n * batchCount values as an int, or std::length_error if the host arrays cannot hold them
+inline int estl_dropin_values(const char *who, int n, int batchCount, long long max_values) {
+    const long long v = (long long)n * batchCount;
+    if (n < 1 || batchCount < 1 || v > max_values)
+        throw std::length_error(std::string(who) + ": " + std::to_string(n) + " x " + std::to_string(batchCount) +
+                                " values do not fit host arrays indexed by int (at most " +
+                                std::to_string(max_values) + ")");
+    return int(v);
+}
+
+// Host-array limits that hold with this project's Complex.h and with the reference's
+// (whose ComplexArray allocates 2 * size in int, Complex.h:160): an interleaved array
+// holds at most INT_MAX / 2 = 2^30 - 1 values; a PlannarizedComplexArray rounds up to
+// tiles of 32 values and counts its scalars in an int, so at most 2^30 - 32 values.
+static const long long kEstlDropinMaxValues = INT_MAX / 2;
+static const long long kEstlMaxPlanarValues = ((long long)INT_MAX / 64) * 32;
+
 template <class D>
 class FFT_HIP;
 
@@ -37,17 +69,13 @@ class FFT_HIP<float> : public FFT<float> {
 
     static std::string getName() { return "HIP FFT (gfx950)"; }
 
-    // largest batch of length-n rows that fits the device's free memory (64-bit query;
-    // the reference's int query overflows on large devices, uOpenCL.h:323-337)
-    static int getMaxBatchSize(int n) {
-        const long long b = estl_max_batch(n, ESTL_PRECISION_FP32, 0);
-        if (b < 1) return 1;
-        return b > INT_MAX ? INT_MAX : int(b);
-    }
+    // FFT_OpenCL_Contiguous.h:43-44, FFT.h:30
+    static int getMaxBatchSize(int n) { return estl_dropin_max_batch(n, kEstlDropinMaxValues); }
 
     FFT_HIP(int iSize, bool iForward = true, int iBatchCount = 1, unsigned flags = 0)
-        : FFT<float>(iSize, iBatchCount), forward(iForward), plan(nullptr) {
-        this->dataInterface = arrays = new DataInterfaceType(iSize * iBatchCount, false);
+        : FFT<float>(iSize, iBatchCount), forward(iForward), arrays(nullptr), plan(nullptr) {
+        const int values = estl_dropin_values("FFT_HIP", iSize, iBatchCount, kEstlDropinMaxValues);
+        this->dataInterface = arrays = new DataInterfaceType(values, false);
         const int rc = estl_plan_create(&plan, iSize, iBatchCount, iForward ? ESTL_FORWARD : ESTL_BACKWARD,
                                         ESTL_PRECISION_FP32, flags);
         if (rc != ESTL_OK) throw std::runtime_error(std::string("FFT_HIP: ") + estl_last_error());
@@ -103,11 +131,12 @@ class FFT_HIP_Planar<float> : public FFT<float> {
     typedef PlannarizedDataInterface<float> DataInterfaceType;
 
     static std::string getName() { return "HIP FFT (gfx950, planarized tiles)"; }
-    static int getMaxBatchSize(int n) { return FFT_HIP<float>::getMaxBatchSize(n); }
+    static int getMaxBatchSize(int n) { return estl_dropin_max_batch(n, kEstlMaxPlanarValues); }
 
     FFT_HIP_Planar(int iSize, bool iForward = true, int iBatchCount = 1, unsigned flags = 0)
-        : FFT<float>(iSize, iBatchCount), plan(nullptr) {
-        this->dataInterface = arrays = new DataInterfaceType(iSize * iBatchCount);
+        : FFT<float>(iSize, iBatchCount), arrays(nullptr), plan(nullptr) {
+        const int values = estl_dropin_values("FFT_HIP_Planar", iSize, iBatchCount, kEstlMaxPlanarValues);
+        this->dataInterface = arrays = new DataInterfaceType(values);
         const int rc = estl_plan_create(&plan, iSize, iBatchCount, iForward ? ESTL_FORWARD : ESTL_BACKWARD,
                                         ESTL_PRECISION_FP32, flags | ESTL_FLAG_PLANAR);
         if (rc != ESTL_OK) throw std::runtime_error(std::string("FFT_HIP_Planar: ") + estl_last_error());
