@@ -28,8 +28,17 @@
 //   * latency (submit -> completion observed) and throughput are recorded per request;
 //     completed ids are delivered through a completion queue.
 // One dispatcher thread admits requests in arrival order; one launcher thread per stream
-// issues their HIP calls (outside the lock, streams in parallel); one completer thread
-// watches the requests' events.
+// (or fewer, ESTL_SERVER_LAUNCHERS) issues their HIP calls (outside the lock, streams in
+// parallel); one completer thread watches the requests' events.
+//
+// Failure handling: a launch that fails after earlier launches of its request were
+// queued leaves the request's memory in place until an event recorded after those
+// launches completes (they may still write into it); only then does the request fail
+// and its blocks return.  An output copy (estl_server_copy_output) records an event on
+// the caller's stream; a release while such copies are queued returns the blocks once
+// they have run.  The host logic is exercised on the CPU, under ThreadSanitizer and
+// AddressSanitizer, over a stand-in runtime with injected launch failures
+// (tests/host/).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,6 +46,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -138,6 +148,12 @@ struct Request {
     Clock::time_point t_submit, t_done;
     hipEvent_t ev = nullptr;     // recorded after the request's last launch
     hipEvent_t ev_in = nullptr;  // recorded on the submitter's stream: inputs are ready
+    // a launch failed after earlier launches of the request were queued: the request
+    // fails with (error, msg) once `ev`, recorded after them, has completed
+    bool launch_failed = false;
+    // estl_server_copy_output copies still queued on callers' streams: the output block
+    // goes back to the arena only once each of these has completed
+    std::vector<hipEvent_t> copies;
     double alg_bytes = 0;
     // resolved at admission (the launcher threads never touch the caches)
     Plan *plan = nullptr;
@@ -177,9 +193,17 @@ class Server {
         if (hipHostMalloc((void **)&host_desc_, sizeof(ConvGroup) * kGroupSlots) != hipSuccess)
             return set_error(ESTL_ERR_NOMEM, "pinned group descriptors");
         for (int i = kGroupSlots - 1; i >= 0; --i) free_slots_.push_back(i);
+        // launcher threads: one per stream, or fewer when ESTL_SERVER_LAUNCHERS asks (launcher
+        // i then issues the work of streams i, i + L, ...; the streams still run
+        // concurrently on the device)
+        nlaunchers_ = nstreams_;
+        if (const char *e = getenv("ESTL_SERVER_LAUNCHERS")) {
+            const int v = atoi(e);
+            if (v >= 1) nlaunchers_ = std::min(v, nstreams_);
+        }
         work_.resize(nstreams_);
-        cv_work_ = std::vector<std::condition_variable>(nstreams_);
-        for (int i = 0; i < nstreams_; ++i) launchers_.emplace_back([this, i] { launch_loop(i); });
+        cv_work_ = std::vector<std::condition_variable>(nlaunchers_);
+        for (int i = 0; i < nlaunchers_; ++i) launchers_.emplace_back([this, i] { launch_loop(i); });
         dispatcher_ = std::thread([this] { dispatch_loop(); });
         completer_ = std::thread([this] { complete_loop(); });
         return ESTL_OK;
@@ -203,7 +227,16 @@ class Server {
         for (auto &kv : reqs_) {
             if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
             if (kv.second.ev_in) (void)hipEventDestroy(kv.second.ev_in);
+            for (hipEvent_t e : kv.second.copies) {
+                (void)hipEventSynchronize(e);
+                (void)hipEventDestroy(e);
+            }
         }
+        for (auto &d : deferred_)
+            for (hipEvent_t e : d.copies) {
+                (void)hipEventSynchronize(e);
+                (void)hipEventDestroy(e);
+            }
         for (auto e : events_) (void)hipEventDestroy(e);
         if (host_desc_) (void)hipHostFree(host_desc_);
     }
@@ -302,10 +335,24 @@ class Server {
         if (it == reqs_.end()) return set_error(ESTL_ERR_INVALID, "unknown request %lld", id);
         const Request &r = it->second;
         if (r.state != DONE) return set_error(ESTL_ERR_INVALID, "request %lld is not done", id);
+        if (r.blocks.empty()) return set_error(ESTL_ERR_INVALID, "request %lld has no output", id);
         if (bytes < 0 || (size_t)bytes > r.blocks[0].second)
             return set_error(ESTL_ERR_INVALID, "copy of %lld bytes exceeds the output (%zu)", bytes, r.blocks[0].second);
-        if (hipMemcpyAsync(dst, r.out, (size_t)bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        // the copy runs later on the caller's stream: an event after it keeps the output
+        // block out of the arena until it has run, whenever the request is released
+        hipEvent_t done = nullptr;
+        if (int rc = get_event(&done)) return rc;
+        if (hipMemcpyAsync(dst, r.out, (size_t)bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+            put_event(done);
             return set_error(ESTL_ERR_HIP, "output copy failed");
+        }
+        if (hipEventRecord(done, st) != hipSuccess) {
+            // the copy is queued but cannot be marked: wait for it here
+            (void)hipStreamSynchronize(st);
+            put_event(done);
+            return ESTL_OK;
+        }
+        it->second.copies.push_back(done);
         return ESTL_OK;
     }
 
@@ -317,7 +364,15 @@ class Server {
         if (r.state == QUEUED || r.state == RUNNING) return set_error(ESTL_ERR_BUSY, "request %lld still in progress", id);
         if (r.users) return set_error(ESTL_ERR_BUSY, "filter %lld is used by %d requests", id, r.users);
         if (r.kind == KIND_FILTER && r.state == DONE) arena_.unhold(r.blocks[0].second);
-        for (auto &b : r.blocks) arena_.release(b.first, b.second);
+        bool copying = false;
+        for (hipEvent_t e : r.copies) copying = copying || hipEventQuery(e) == hipErrorNotReady;
+        if (copying) {
+            // output copies still queued read these blocks: the completer returns them
+            deferred_.push_back(Deferred{std::move(r.blocks), std::move(r.copies)});
+        } else {
+            for (auto &b : r.blocks) arena_.release(b.first, b.second);
+            for (hipEvent_t &e : r.copies) put_event(e);
+        }
         put_event(r.ev);
         put_event(r.ev_in);
         reqs_.erase(it);
@@ -377,6 +432,26 @@ class Server {
   private:
     static Clock::duration to_duration(double s) {
         return std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(std::max(0.0, s)));
+    }
+
+    // Blocks of released requests whose output copies were still queued: returned to the
+    // arena once every copy has run (completer thread).  True if any came back.
+    bool return_deferred() {
+        std::lock_guard<std::mutex> lk(mu_);
+        bool any = false;
+        for (auto it = deferred_.begin(); it != deferred_.end();) {
+            bool pending = false;
+            for (hipEvent_t e : it->copies) pending = pending || hipEventQuery(e) == hipErrorNotReady;
+            if (pending) {
+                ++it;
+                continue;
+            }
+            for (auto &b : it->blocks) arena_.release(b.first, b.second);
+            for (hipEvent_t &e : it->copies) put_event(e);
+            it = deferred_.erase(it);
+            any = true;
+        }
+        return any;
     }
 
     // event pool (under mu_)
@@ -564,7 +639,7 @@ class Server {
         grouped_ += G;
         ++launching_;
         work_[r.stream].push_back(r.id);
-        cv_work_[r.stream].notify_one();
+        cv_work_[r.stream % nlaunchers_].notify_one();
         return true;
     }
 
@@ -845,18 +920,33 @@ class Server {
             }
             ++launching_;
             work_[r.stream].push_back(r.id);
-            cv_work_[r.stream].notify_one();
+            cv_work_[r.stream % nlaunchers_].notify_one();
         }
     }
 
-    // Launcher of stream s: the HIP calls of each admitted request, outside the lock.
-    void launch_loop(int s) {
+    // Launcher i: the HIP calls of each admitted request of its streams (i, i + L, ...),
+    // outside the lock, taking its streams in turn.
+    void launch_loop(int li) {
         (void)hipSetDevice(device_);
-        const hipStream_t st = streams_[s];
         std::unique_lock<std::mutex> lk(mu_);
+        int turn = li;
+        // this launcher's m streams li + j L, j < m, searched from `turn` on: one with
+        // work, or -1
+        const int m = (nstreams_ - li + nlaunchers_ - 1) / nlaunchers_;
+        auto next_stream = [&]() -> int {
+            const int t0 = (turn - li) / nlaunchers_;
+            for (int j = 0; j < m; ++j) {
+                const int s = li + ((t0 + j) % m) * nlaunchers_;
+                if (!work_[s].empty()) return s;
+            }
+            return -1;
+        };
         while (true) {
-            cv_work_[s].wait(lk, [&] { return stop_ || !work_[s].empty(); });
-            if (work_[s].empty()) return;  // stopping, nothing left to launch
+            int s = -1;
+            cv_work_[li].wait(lk, [&] { return stop_ || (s = next_stream()) >= 0; });
+            if (s < 0) return;  // stopping, nothing left to launch
+            turn = s + nlaunchers_ < nstreams_ ? s + nlaunchers_ : li;
+            const hipStream_t st = streams_[s];
             Request &r = reqs_.at(work_[s].front());
             work_[s].pop_front();
             std::vector<Request *> fol;  // a group's followers (running: not erased meanwhile)
@@ -866,11 +956,22 @@ class Server {
             if (hipStreamWaitEvent(st, r.ev_in, 0) != hipSuccess)
                 rc = set_error(ESTL_ERR_HIP, "stream wait on the input event failed");
             if (!rc) rc = fol.empty() ? launch(r, st) : launch_group(r, fol, st);
-            if (!rc && hipEventRecord(r.ev, st) != hipSuccess) rc = set_error(ESTL_ERR_HIP, "event record failed");
+            bool marked = false;  // r.ev follows everything of r queued on st
+            if (!rc) {
+                if (hipEventRecord(r.ev, st) == hipSuccess) marked = true;
+                else rc = set_error(ESTL_ERR_HIP, "event record failed");
+            }
             const std::string err = rc ? estl_last_error() : std::string();
+            if (rc && !marked) {
+                // launches of r queued before the failure may still run and write into
+                // r's blocks (or read its descriptor slot): r keeps them until those
+                // launches are done -- marked by an event, or else waited for here
+                if (hipEventRecord(r.ev, st) == hipSuccess) marked = true;
+                else (void)hipStreamSynchronize(st);
+            }
             lk.lock();
             --launching_;
-            if (rc) {
+            if (rc && !marked) {
                 if (r.kind == KIND_FILTER) arena_.unhold(r.blocks[0].second);
                 if (fol.empty())
                     fail_locked(r, rc, err);
@@ -878,6 +979,11 @@ class Server {
                     fail_group_locked(r, rc, err);
                 cv_dispatch_.notify_one();
                 continue;
+            }
+            if (rc) {  // fails when r.ev completes (complete_loop)
+                r.launch_failed = true;
+                r.error = rc;
+                r.msg = err;
             }
             inflight_.push_back(r.id);
             peak_inflight_ = std::max(peak_inflight_, inflight_.size());
@@ -894,7 +1000,7 @@ class Server {
                 watch.clear();
                 for (long long id : inflight_) watch.emplace_back(id, reqs_.at(id).ev);
             }
-            bool any = false;
+            bool any = return_deferred();
             for (auto &w : watch) {
                 const hipError_t q = hipEventQuery(w.second);
                 if (q == hipErrorNotReady) continue;
@@ -902,13 +1008,16 @@ class Server {
                 Request &r = reqs_.at(w.first);
                 inflight_.erase(std::find(inflight_.begin(), inflight_.end(), w.first));
                 put_event(r.ev_in);
-                if (q != hipSuccess) {
+                if (q != hipSuccess || r.launch_failed) {
+                    // a device error, or a launch failure whose earlier launches have now run
                     if (r.kind == KIND_FILTER) arena_.unhold(r.blocks[0].second);
-                    const std::string msg = std::string("device error: ") + hipGetErrorString(q);
+                    const int code = r.launch_failed ? r.error : ESTL_ERR_HIP;
+                    const std::string msg =
+                        r.launch_failed ? r.msg : std::string("device error: ") + hipGetErrorString(q);
                     if (r.followers.empty())
-                        fail_locked(r, ESTL_ERR_HIP, msg);
+                        fail_locked(r, code, msg);
                     else
-                        fail_group_locked(r, ESTL_ERR_HIP, msg);
+                        fail_group_locked(r, code, msg);
                 } else {
                     r.t_done = Clock::now();
                     r.state = DONE;
@@ -948,7 +1057,7 @@ class Server {
         }
     }
 
-    int device_ = 0, nstreams_ = 1, max_inflight_ = 2;
+    int device_ = 0, nstreams_ = 1, nlaunchers_ = 1, max_inflight_ = 2;
     long max_queue_ = 1 << 16;
     Arena arena_;
     std::vector<hipStream_t> streams_;
@@ -977,6 +1086,11 @@ class Server {
     std::map<std::tuple<long, long, int>, std::unique_ptr<Plan>> half_plans_;  // (n / 2, batch, direction)
     ConvGroup *host_desc_ = nullptr;  // kGroupSlots pinned descriptors
     std::vector<int> free_slots_;
+    struct Deferred {
+        std::vector<std::pair<size_t, size_t>> blocks;
+        std::vector<hipEvent_t> copies;
+    };
+    std::vector<Deferred> deferred_;  // released blocks waiting for their output copies
     long long groups_ = 0, grouped_ = 0;
 };
 

@@ -317,10 +317,14 @@ int estl_server_wait(estl_server_t srv, long long request_id, double timeout_s);
  * first, waiting up to timeout_s for the first; returns how many (0 on timeout). */
 int estl_server_next_done(estl_server_t srv, long long *request_ids, int max_ids, double timeout_s);
 
-/* Copy `bytes` of a DONE request's output to d_dst on `stream` (asynchronous). */
+/* Copy `bytes` of a DONE request's output to d_dst on `stream` (asynchronous).  The
+ * request may be released right after: its output block returns to the arena only once
+ * the copies queued here have run. */
 int estl_server_copy_output(estl_server_t srv, long long request_id, void *d_dst, long long bytes, void *stream);
 
-/* Return the request's output to the arena (after DONE or FAILED). */
+/* Return the request's output to the arena (after DONE or FAILED).  A FAILED request
+ * whose launches were partly queued has already waited for them (it turns FAILED only
+ * then), so its memory is never reused under a launch still running. */
 int estl_server_release(estl_server_t srv, long long request_id);
 
 /* Wait until nothing is queued or running. */
