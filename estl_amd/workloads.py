@@ -80,7 +80,7 @@ def fill_uneven(batch: UnevenBatch, buf, torch) -> None:
 @dataclass
 class ConvRequests:
     n: np.ndarray          # int64, row length 2^U{log2n_min..log2n_max}
-    rows: np.ndarray       # int64, rows per request, U{1..rows_max} capped at max_values / n
+    rows: np.ndarray       # int64, rows per request, U{1..rows_max} (capped at max_values / n)
     k: np.ndarray          # int64, filter length U{k_min..k_max}
     flags: np.ndarray      # int64, 0 or correlation (correlate_fraction of requests)
     bank: np.ndarray       # int64, index into a bank of `bank_size` filters (bank mode)
@@ -93,11 +93,13 @@ class ConvRequests:
 
 
 def conv_requests(count: int = 1000, seed: int = 4, log2n=(10, 22), k_range=(16, 8192), rows_max: int = 256,
-                  max_values: int = 1 << 22, correlate_fraction: float = 0.25, bank_size: int = 32) -> ConvRequests:
+                  max_values: int = 1 << 30, correlate_fraction: float = 0.25, bank_size: int = 32) -> ConvRequests:
     """The served convolution stream: request i convolves rows[i] real rows of n[i]
     values with its own k[i]-value filter (or, in bank mode, with filter bank[i] of a
     fixed bank); arrivals are a Poisson process (exponential gaps of unit mean, scaled
-    by 1 / rate by the driver)."""
+    by 1 / rate by the driver).  Rows are U{1..rows_max} at every length: the default
+    cap of 2^30 values (4 GiB of FP32 input) is what 256 rows of 2^22 reach, so it only
+    binds for smaller `max_values`."""
     import estl_amd
     ln = log2n[0] + (stream(seed, count) % np.uint64(log2n[1] - log2n[0] + 1)).astype(np.int64)
     span = np.uint64(k_range[1] - k_range[0] + 1)

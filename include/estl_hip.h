@@ -271,7 +271,7 @@ int estl_conv_destroy(estl_conv_t conv);
 
 int estl_server_create(estl_server_t *srv, long long hbm_budget_bytes, int streams);
 
-/* Batched complex FFT of `batch` interleaved rows of n (FFT_OpenCL_Contiguous.h:128-161
+/* Batched complex FFT of `batch` interleaved rows of n (FFT_OpenCL_Contiguous.h:97-142
  * semantics); the output (batch * n complex values) lives in the arena until release. */
 int estl_server_submit_fft(estl_server_t srv, long n, long batch, int direction, const void *d_in,
                            void *input_stream, long long *request_id);

@@ -97,6 +97,47 @@ def test_mix_equals_alone_bitwise(gpu):
     srv.close()
 
 
+def test_large_requests_beside_small_ones_bitwise(gpu):
+    """Requests of the served stream's largest shapes -- 64 and 256 rows of 2^22 values
+    (1 GiB and 4 GiB of input, up to 2^30 values) -- served beside small requests that
+    arrive while they run: every result equals its request run alone, bit for bit; one
+    sampled row of the 64-row request against the FP64 convolution theorem."""
+    import torch
+    cases = [("conv", 2 ** 22, 64, 777, 0), ("conv", 2 ** 10, 7, 16, 0), ("conv", 2 ** 16, 3, 5000, 0),
+             ("fft", 4096, 8, 0, estl_amd.FORWARD), ("conv", 2 ** 22, 256, 8192, estl_amd.CONV_CORRELATE),
+             ("conv", 2 ** 13, 256, 300, estl_amd.CONV_CORRELATE), ("conv", 2 ** 18, 1, 64, 0)]
+    print("[large] inputs", flush=True)
+    inputs = [make_inputs(torch, gpu, c, 90 + i) for i, c in enumerate(cases)]
+    refs = []
+    for c, (x, h) in zip(cases, inputs):
+        refs.append(alone(torch, gpu, c, x, h))
+    # FP64 check of one row of the 64-row request (convolution theorem, numpy)
+    n, k = 2 ** 22, 777
+    x0 = inputs[0][0][5 * n:6 * n].cpu().numpy().astype(np.float64)
+    h0 = np.zeros(n)
+    h0[:k] = inputs[0][1].cpu().numpy()
+    want = np.fft.ifft(np.fft.fft(x0) * np.fft.fft(h0)).real
+    got = refs[0][5 * n:6 * n].cpu().numpy()
+    assert oracle.rel_l2(got, want) <= 2e-6 * math.log2(n)
+    srv = estl_amd.Server(24 << 30, streams=4)
+    print("[large] served", flush=True)
+    ids = []
+    for rep in range(2):
+        for i, c in enumerate(cases):
+            ids.append((i, submit(srv, c, *inputs[i])))
+    srv.drain(300)
+    for i, rid in ids:
+        srv.wait(rid, 60)
+        y = result(srv, rid, cases[i], torch)
+        assert torch.equal(y, refs[i]), (cases[i], rid)
+        del y
+        srv.release(rid)
+    st = srv.stats()
+    assert st["completed"] == len(ids) and st["failed"] == 0 and st["rejected"] == 0
+    assert st["arena_used_bytes"] == 0
+    srv.close()
+
+
 def test_served_results_match_oracle(gpu):
     import torch
     srv = estl_amd.Server(256 << 20, streams=2)
