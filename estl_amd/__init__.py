@@ -71,6 +71,7 @@ def lib():
                                               ctypes.POINTER(ci)])
     _sig(L, "estl_plan_destroy", ci, [vp])
     _sig(L, "estl_execute_device", ci, [vp, vp, vp, vp])
+    _sig(L, "estl_execute_device_noalloc", ci, [vp, vp, vp, vp])
     _sig(L, "estl_execute_host", ci, [vp, vp, vp])
     _sig(L, "estl_max_batch", ll, [cl, ci, ll])
     _sig(L, "estl_plan_workspace_bytes", ll, [vp])
@@ -170,6 +171,14 @@ class Plan:
         pin = d_in if isinstance(d_in, int) else d_in.data_ptr()
         pout = d_out if isinstance(d_out, int) else d_out.data_ptr()
         check(lib().estl_execute_device(self._h, pin, pout, _stream_ptr(stream)))
+
+    def execute_noalloc(self, d_in, d_out, stream=None) -> None:
+        """The same with no device allocation: scratch from the library's static 64 MiB
+        array of this GPU (plans of at most 2^23 values; calls overlapping on one GPU
+        race)."""
+        pin = d_in if isinstance(d_in, int) else d_in.data_ptr()
+        pout = d_out if isinstance(d_out, int) else d_out.data_ptr()
+        check(lib().estl_execute_device_noalloc(self._h, pin, pout, _stream_ptr(stream)))
 
     def execute_fp16(self, d_in, in_scale, d_out, out_scale, stream=None) -> None:
         """FP16-storage plan: binary16 (re, im) buffers with one FP32 scale per row."""

@@ -31,6 +31,7 @@ SOURCES = [("kernels.hip", [], "kernels"),
            ("workqueue.hip", [], "workqueue"),
            ("fp16.hip", [], "fp16"),
            ("conv.hip", [], "conv"),
+           ("noalloc.hip", [], "noalloc"),
            ("estl_c.cpp", [], "estl_c"),
            ("tune.cpp", [], "tune"),
            ("fp16_c.cpp", [], "fp16_c"),

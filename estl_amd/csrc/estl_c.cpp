@@ -403,7 +403,9 @@ int estl_init(int device) {
         return set_error(ESTL_ERR_INVALID, "device %d out of range (%d devices)", device, count);
     HIP_TRY(hipSetDevice(device));
     g_device = device;
-    return ESTL_OK;
+    // the no-allocation entry point's scratch address, set up once per device here
+    float2 *noalloc = nullptr;
+    return noalloc_scratch(&noalloc);
 }
 
 int estl_plan_create(estl_plan_t *plan, long n, long batch, int direction, int precision, unsigned flags) {

@@ -81,6 +81,9 @@ int ensure_scratch(Plan *p, int which);
 // scratch pair of stream st (allocated on first use; second only if `two`)
 int stream_scratch(Plan *p, hipStream_t st, bool two, float2 **ws0, float2 **ws1);
 void destroy_plan(Plan *p);
+// device address of the no-allocation entry point's static scratch on the current device
+// (noalloc.hip; looked up once per device)
+int noalloc_scratch(float2 **out);
 void free_stream_scratch(Plan *p);  // the per-stream scratch of a plan not destroyed by destroy_plan
 std::string describe_plan(const Plan *p);
 int plan_launches(const Plan *p);

@@ -111,6 +111,17 @@ int estl_plan_destroy(estl_plan_t plan);
  * (FFT_OpenCL_Contiguous.h:108-124). */
 int estl_execute_device(estl_plan_t plan, const void *d_in, void *d_out, void *stream);
 
+/* estl_execute_device without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  A multi-pass plan's scratch comes from a
+ * static 64 MiB device array per GPU instead of a per-stream buffer allocated on first
+ * use, so the plan's n * batch may be at most 2^23 values (2^22 for an in-place plan of
+ * an odd number of passes); larger plans are rejected with ESTL_ERR_INVALID before any
+ * device work.  FP32 plans only.  All calls on one GPU share that array: calls that
+ * overlap in time on one GPU (several streams or threads) race, and the caller must
+ * serialise them.  Same results as estl_execute_device, bit for bit. */
+int estl_execute_device_noalloc(estl_plan_t plan, const void *d_in, void *d_out, void *stream);
+
 /* ---------------------------------------------------------------- 2-D transforms
  * `batch` arrays of ny rows of nx complex64, x fastest (FFT2D's data[y * xDim + x],
  * FFT.h:54-84, which runs one host round trip per row and per column).  Rows are one
