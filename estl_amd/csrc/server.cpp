@@ -943,7 +943,9 @@ class Server {
         };
         while (true) {
             int s = -1;
-            cv_work_[li].wait(lk, [&] { return stop_ || (s = next_stream()) >= 0; });
+            // work first: admitted requests are launched even while stopping (the
+            // completer waits for every one of them)
+            cv_work_[li].wait(lk, [&] { return (s = next_stream()) >= 0 || stop_; });
             if (s < 0) return;  // stopping, nothing left to launch
             turn = s + nlaunchers_ < nstreams_ ? s + nlaunchers_ : li;
             const hipStream_t st = streams_[s];
