@@ -29,6 +29,7 @@ CONV_CORRELATE = 0x10
 CONV_NORMALIZE_KERNEL = 0x20
 CONV_CENTER = 0x40
 CONV_FP16 = 0x80
+CONV_FULL_LENGTH = 0x100
 
 OK = 0
 ERR_INVALID = -1

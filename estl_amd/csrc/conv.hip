@@ -324,6 +324,104 @@ static long conv_resident(K kernel, int threads) {
     return co_resident_blocks(reinterpret_cast<const void *>(kernel), threads);
 }
 
+// Overlap-save circular convolution of rows of n > L values with a filter of K <= L taps
+// through L-point transforms (one launch, one HBM read of each window and one write of
+// each output).  Window w of the batch (row r = w / W, block b = w mod W) holds the L
+// input values x_r[(b M - pre + i) mod n], i < L; its circular L-point convolution with
+// the filter equals the circular n-point one for i in [pre, pre + M): pre = K - 1 for a
+// convolution, 0 for a correlation (the filter then reaches forward), M = L - K + 1.
+// Those outputs are positions p = b M + i - pre of the row's result, stored at
+// (p + oshift) mod n for p < n (oshift undoes the filter rotation of ESTL_CONV_CENTER,
+// which rotates the output instead).  Two windows (any two, of any rows) share one
+// complex transform, re and im; the spectrum is the filter's L-point spectrum.
+struct OlsArgs {
+    long rows;    // real rows of n values
+    long W;       // windows per row, ceil(n / M)
+    long oshift;  // output rotation, in [0, n)
+    int log2n;
+    int M;        // valid outputs per window
+    int pre;      // first valid position in a window
+};
+
+template <int LOG2L, bool CORR, typename R>
+__global__ void __launch_bounds__(BlockGeom<LOG2L>::TPT)
+k_conv_ols(const R *__restrict__ x, R *__restrict__ y, OlsArgs a, const float2 *__restrict__ spec,
+           const float2 *__restrict__ tw, float scale) {
+    using G = BlockGeom<LOG2L>;
+    constexpr int E = G::E, TPT = G::TPT;
+    __shared__ float2 lds[Lds<kLdsPad>::size(G::N)];
+    const long windows = a.rows * a.W;
+    const long pairs = (windows + 1) / 2;
+    const long nmask = (1L << a.log2n) - 1;
+    for (long pr = blockIdx.x; pr < pairs; pr += gridDim.x) {  // uniform across the workgroup
+        const int tl = opaque_lane(threadIdx.x);
+        const long w0 = 2 * pr, w1 = w0 + 1;
+        const bool two = w1 < windows;
+        const long r0 = w0 / a.W, b0 = w0 - r0 * a.W;
+        const long r1 = two ? w1 / a.W : r0, b1 = two ? w1 - r1 * a.W : b0;
+        const R *x0 = x + (r0 << a.log2n), *x1 = x + (r1 << a.log2n);
+        const long s0 = b0 * a.M - a.pre, s1 = b1 * a.M - a.pre;
+        float2 v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const long i = tl + TPT * e;
+            v[e] = make_float2(ld_real(x0 + ((s0 + i) & nmask)), two ? ld_real(x1 + ((s1 + i) & nmask)) : 0.0f);
+        }
+        block_fft<LOG2L, -1, kLdsPad>(v, lds, tl, tw, true);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const float2 hk = spec[tl + TPT * e];
+            v[e] = CORR ? cmulc(v[e], hk) : cmul(v[e], hk);
+        }
+        block_fft<LOG2L, +1, kLdsPad>(v, lds, tl, tw, true);
+        const long n = nmask + 1;
+        R *y0 = y + (r0 << a.log2n), *y1 = y + (r1 << a.log2n);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int q = tl + TPT * e - a.pre;  // position in the window's valid outputs
+            if (q < 0 || q >= a.M) continue;
+            const long p0 = b0 * a.M + q, p1 = b1 * a.M + q;
+            if (p0 < n) st_real(y0 + ((p0 + a.oshift) & nmask), v[e].x * scale);
+            if (two && p1 < n) st_real(y1 + ((p1 + a.oshift) & nmask), v[e].y * scale);
+        }
+    }
+}
+
+template <int LOG2L, bool CORR, typename R>
+static hipError_t launch_conv_ols_t(const void *x, void *y, const OlsArgs &a, const float2 *spec, const float2 *tw,
+                                    hipStream_t st) {
+    using G = BlockGeom<LOG2L>;
+    auto kernel = k_conv_ols<LOG2L, CORR, R>;
+    const long resident = conv_resident(kernel, G::TPT);
+    const long pairs = (a.rows * a.W + 1) / 2;
+    long blocks = pairs;
+    if (blocks > 4 * resident) blocks = 4 * resident;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(G::TPT), 0, st, static_cast<const R *>(x),
+                       static_cast<R *>(y), a, spec, tw, 1.0f / (float)G::N);
+    return hipGetLastError();
+}
+
+// x, y: rows of n FP32 (fp16 = false) or binary16 reals; spectrum and table of L = 2^13
+hipError_t launch_conv_ols(bool fp16, bool corr, const void *x, void *y, long rows, int log2n, long k, long oshift,
+                           const float2 *spec, const float2 *tw, hipStream_t st) {
+    constexpr int LOG2L = kOlsLog2;
+    const long n = 1L << log2n;
+    const long M = (1L << LOG2L) - k + 1;
+    if (log2n <= LOG2L || k < 1 || M < 1 || rows < 1) return hipErrorInvalidValue;
+    OlsArgs a;
+    a.rows = rows;
+    a.W = (n + M - 1) / M;
+    a.oshift = ((oshift % n) + n) % n;
+    a.log2n = log2n;
+    a.M = (int)M;
+    a.pre = corr ? 0 : (int)(k - 1);
+    if (fp16)
+        return corr ? launch_conv_ols_t<LOG2L, true, _Float16>(x, y, a, spec, tw, st)
+                    : launch_conv_ols_t<LOG2L, false, _Float16>(x, y, a, spec, tw, st);
+    return corr ? launch_conv_ols_t<LOG2L, true, float>(x, y, a, spec, tw, st)
+                : launch_conv_ols_t<LOG2L, false, float>(x, y, a, spec, tw, st);
+}
+
 template <int LOG2N, bool CORR>
 static hipError_t launch_conv_rows_t(const float *x, float *y, long rows, const float2 *spec, const float2 *tw,
                                      float scale, hipStream_t st, const ConvGroup *group) {

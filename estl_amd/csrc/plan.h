@@ -11,6 +11,7 @@
 #include <utility>
 #include <vector>
 
+#include "../../include/estl_hip.h"
 #include "internal.h"
 
 namespace estl {
@@ -109,13 +110,22 @@ struct ConvShape {
     // k_conv_half_merge, taken when the row's input and output are 8-byte aligned
     bool half = false;
     Plan half_fwd, half_inv;
+    const float2 *tw_ols = nullptr;  // block table of the overlap-save windows (n > 2^13)
 };
 bool conv_half_path(long n, long rows, int precision);
+// Rows longer than an overlap-save window (n > 2^kOlsLog2) convolved with a filter of
+// at most kOlsMaxTaps taps run overlap-save unless ESTL_CONV_FULL_LENGTH asks for
+// whole-row transforms: windows of 2^kOlsLog2 values, one launch, the filter's spectrum
+// of that length (conv.hip k_conv_ols)
+inline bool conv_ols_path(long n, long k, unsigned flags) {
+    return n > (1L << kOlsLog2) && k >= 1 && k <= kOlsMaxTaps && !(flags & ESTL_CONV_FULL_LENGTH);
+}
 int conv_run_group_half(const Plan &fwd, const Plan &inv, const ConvGroup *d_group, int members, long n, bool corr,
                         float2 *z, float2 *ws, hipStream_t st);
 struct ConvBuffers {
     float2 *spec = nullptr, *dc = nullptr;
     float2 *scratch[2] = {nullptr, nullptr};
+    long k = 0;  // taps of the filter whose spectrum `spec` holds (picks the pipeline)
 };
 int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precision = 0);
 size_t conv_footprint(const ConvShape &c);  // bytes of ConvBuffers

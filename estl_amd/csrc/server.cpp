@@ -273,6 +273,10 @@ class Server {
             if (it->second.state == FAILED) return give_back(set_error(ESTL_ERR_INVALID, "filter %lld failed", r.filter));
             filter = &it->second;
             r.n = filter->n;
+            r.k = filter->k;  // its spectrum's filter length (picks the pipeline)
+            // the registration's whole-row choice and centring: an overlap-save spectrum
+            // leaves the centring rotation to the convolution's output stores
+            r.flags |= filter->flags & (ESTL_CONV_FULL_LENGTH | ESTL_CONV_CENTER);
             if ((filter->flags & ESTL_CONV_FP16) != (r.flags & ESTL_CONV_FP16))
                 return give_back(
                     set_error(ESTL_ERR_INVALID, "filter %lld and the request differ in FP16 storage", r.filter));
@@ -531,6 +535,7 @@ class Server {
     bool groupable(const Request &r) const {
         if (r.kind != KIND_CONV && r.kind != KIND_CONV_REG) return false;
         if (r.flags & ESTL_CONV_NORMALIZE_KERNEL) return false;
+        if (conv_ols_path(r.n, r.k, r.flags)) return false;  // one overlap-save launch alone
         if (half_row(r) && (r.n <= (2L << kMaxRowsLog2) || (reinterpret_cast<uintptr_t>(r.in) & 7))) return false;
         if (r.n < 2 || r.rows < 1) return false;
         if (r.kind == KIND_CONV_REG) {
@@ -753,14 +758,15 @@ class Server {
                 else
                     sizes.push_back(sizeof(float) * (size_t)r.n * (size_t)r.rows);
                 // two scratch buffers: the pipeline's (multi-pass sizes) and, with an
-                // inline filter, the spectrum transform's (it borrows the first)
-                const size_t sb = r.kind == KIND_CONV ? conv_scratch_bytes(*c) : c->scratch_bytes;
+                // inline filter, the spectrum transform's (it borrows the first); an
+                // overlap-save request needs none, and a spectrum of its window length
+                const size_t sb = request_scratch(r, *c);
                 if (sb) {
                     sizes.push_back(sb);
                     sizes.push_back(sb);
                 }
                 if (r.kind == KIND_CONV) {
-                    sizes.push_back(c->spec_bytes);
+                    sizes.push_back(spectrum_bytes(r, *c));
                     sizes.push_back(256);
                 }
                 r.alg_bytes = ((r.flags & ESTL_CONV_FP16) ? 4.0 : 8.0) * r.n * r.rows + (r.kind == KIND_CONV ? 4.0 * r.k : 0.0);
@@ -770,15 +776,27 @@ class Server {
                 ConvShape *c = conv_shape(r.n, 1, r.flags);
                 if (!c) return fail();
                 r.shape = c;
-                sizes.push_back(c->spec_bytes);
+                sizes.push_back(spectrum_bytes(r, *c));
                 sizes.push_back(256);
-                if (spec_scratch(*c)) sizes.push_back(spec_scratch(*c));
+                if (spec_scratch(*c) && !conv_ols_path(r.n, r.k, r.flags)) sizes.push_back(spec_scratch(*c));
                 r.alg_bytes = 0;
                 return true;
             }
         }
         why = "unknown request kind";
         return false;
+    }
+
+    // bytes of each scratch buffer of a convolution request (0: none)
+    static size_t request_scratch(const Request &r, const ConvShape &c) {
+        if (conv_ols_path(r.n, r.k, r.flags)) return 0;
+        return r.kind == KIND_CONV ? conv_scratch_bytes(c) : c.scratch_bytes;
+    }
+    // bytes of the filter spectrum an inline filter or a registration computes: n bins,
+    // or an overlap-save window's; FP16 storage keeps one more value (the L1 bound)
+    static size_t spectrum_bytes(const Request &r, const ConvShape &c) {
+        if (!conv_ols_path(r.n, r.k, r.flags)) return c.spec_bytes;
+        return sizeof(float2) * (size_t)((1L << kOlsLog2) + ((r.flags & ESTL_CONV_FP16) ? 1 : 0));
     }
 
     static size_t fp16_out_scales_offset(long n, long rows) {
@@ -803,8 +821,9 @@ class Server {
         }
         ConvShape *c = r.shape;
         ConvBuffers b;
+        b.k = r.k;  // an inline filter's taps, or the registered filter's
         size_t next = 1;
-        const size_t sb = r.kind == KIND_CONV ? conv_scratch_bytes(*c) : c->scratch_bytes;
+        const size_t sb = request_scratch(r, *c);
         if (sb) {
             b.scratch[0] = ptr(next++);
             b.scratch[1] = ptr(next++);

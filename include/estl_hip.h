@@ -55,6 +55,9 @@ extern "C" {
 #define ESTL_CONV_NORMALIZE_KERNEL 0x20u /* divide by the kernel sum K^[0], Convolution.h:141 */
 #define ESTL_CONV_CENTER 0x40u           /* rotate the filter by (K+1)/2, Convolution.h:113-120 */
 #define ESTL_CONV_FP16 0x80u             /* server: binary16 rows with per-row scales         */
+#define ESTL_CONV_FULL_LENGTH 0x100u     /* transform whole rows even where overlap-save      */
+                                         /* windows would serve (rows longer than 2^13 with   */
+                                         /* filters of at most 4097 taps)                     */
 
 typedef struct estl_plan_s *estl_plan_t;
 typedef struct estl_ubatch_s *estl_ubatch_t;

@@ -234,7 +234,7 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precisi
     if (precision != ESTL_PRECISION_FP32 && precision != ESTL_PRECISION_FP16)
         return set_error(ESTL_ERR_UNSUPPORTED, "convolution precision %d not supported", precision);
     if (rows < 1) return set_error(ESTL_ERR_INVALID, "rows=%ld must be >= 1", rows);
-    const unsigned known = ESTL_CONV_CORRELATE | ESTL_CONV_NORMALIZE_KERNEL | ESTL_CONV_CENTER;
+    const unsigned known = ESTL_CONV_CORRELATE | ESTL_CONV_NORMALIZE_KERNEL | ESTL_CONV_CENTER | ESTL_CONV_FULL_LENGTH;
     if (flags & ~known) return set_error(ESTL_ERR_INVALID, "unknown convolution flags 0x%x", flags);
     const bool h = precision == ESTL_PRECISION_FP16;
     c->precision = precision;
@@ -259,6 +259,22 @@ size_t conv_scratch_bytes(const ConvShape &c) {
     return c.scratch_bytes > spec_need ? c.scratch_bytes : spec_need;
 }
 
+// The overlap-save path (conv_ols_path) keeps a spectrum of the window length, 2^kOlsLog2
+// values (plus the FP16 L1 pair): here the window holds the filter's taps themselves,
+// (h_i, 0) zero-padded, and the convolution builds the n-point spectrum from them when it
+// runs -- the same buffer sizes and the same data flow as the library's.
+static void ols_taps(const float *h, long k, float2 *spec) {
+    const long L = 1L << kOlsLog2;
+    for (long i = 0; i < L; ++i) spec[i] = {i < k ? h[i] : 0.0f, 0.0f};
+}
+static void ols_spectrum(const ConvShape &c, const float2 *taps, long k, std::vector<float2> &spec) {
+    std::vector<float> h(k);
+    for (long i = 0; i < k; ++i) h[i] = taps[i].x;
+    const long shift = (c.flags & ESTL_CONV_CENTER) ? ((k + 1) / 2) % c.n : 0;
+    spec.resize(c.n);
+    filter_spectrum(h.data(), k, shift, c.n, (c.flags & ESTL_CONV_NORMALIZE_KERNEL) != 0, spec.data());
+}
+
 int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k, hipStream_t st) {
     if (!h || k < 1) return set_error(ESTL_ERR_INVALID, "filter needs k >= 1 values");
     const long shift = (c.flags & ESTL_CONV_CENTER) ? ((k + 1) / 2) % c.n : 0;
@@ -266,6 +282,13 @@ int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k
     const bool fp16 = c.precision == ESTL_PRECISION_FP16;
     const long n = c.n;
     float2 *spec = b.spec, *scr = b.scratch[0];
+    if (conv_ols_path(n, k, c.flags)) {
+        const long L = 1L << kOlsLog2;
+        return launch(st, "ols filter", [=] {
+            ols_taps(h, k, spec);
+            if (fp16) spec[L] = {l1_of(h, k), l1_of(h, k)};
+        });
+    }
     if (c.spec_plan.kind == Plan::PASSES) {
         if (!scr) return set_error(ESTL_ERR_INVALID, "the filter spectrum of n=%ld needs scratch", n);
         // first pass into scratch 0, the last into the spectrum
@@ -291,6 +314,15 @@ int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y,
     const long n = c.n, rows = c.rows;
     const bool corr = c.corr;
     const float2 *spec = b.spec;
+    if (conv_ols_path(n, b.k, c.flags)) {
+        const ConvShape *cp = &c;
+        const long k = b.k;
+        return launch(st, "conv ols", [=] {
+            std::vector<float2> s;
+            ols_spectrum(*cp, spec, k, s);
+            for (long r = 0; r < rows; ++r) conv_row(x + r * n, n, s.data(), corr, y + r * n);
+        });
+    }
     if (c.fused)
         return launch(st, "conv rows", [=] {
             for (long r = 0; r < rows; ++r) conv_row(x + r * n, n, spec, corr, y + r * n);
@@ -310,7 +342,8 @@ int conv_run_fp16(const ConvShape &c, const ConvBuffers &b, const void *x16, con
                   hipStream_t st) {
     const long n = c.n, rows = c.rows;
     const float2 *spec = b.spec;
-    return launch(st, "conv fp16", [=] { conv_rows16(x16, s_in, n, rows, spec[n].y, y16, s_out); });
+    const long l1_at = conv_ols_path(n, b.k, c.flags) ? (1L << kOlsLog2) : n;
+    return launch(st, "conv fp16", [=] { conv_rows16(x16, s_in, n, rows, spec[l1_at].y, y16, s_out); });
 }
 
 // a group descriptor as the kernels see it: read from device memory when the launch runs

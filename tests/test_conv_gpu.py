@@ -114,8 +114,9 @@ def test_spectrum_first_pass_skip_is_exact(gpu, n, k, period):
     x = oracle.fill_uniform(rows * n, 50).astype(np.float32).reshape(rows, n)
     h = oracle.fill_uniform(k, 51).astype(np.float32)
     hpad = np.concatenate([h, np.zeros(period + 1 - k, np.float32)])
-    y = conv_gpu(torch, gpu, x, h)
-    ypad = conv_gpu(torch, gpu, x, hpad)
+    full = estl_amd.CONV_FULL_LENGTH  # whole-row transforms (their spectrum plan)
+    y = conv_gpu(torch, gpu, x, h, full)
+    ypad = conv_gpu(torch, gpu, x, hpad, full)
     assert np.array_equal(y, ypad)
     x64 = x.astype(np.float64)
     ref = np.real(np.fft.ifft(np.fft.fft(x64, axis=1) * np.fft.fft(folded(h.astype(np.float64), n))[None, :], axis=1))
@@ -147,9 +148,10 @@ def test_single_row_half_length(gpu, n, k, flags):
     H = np.fft.fft(hp)
     X = np.fft.fft(x.astype(np.float64), axis=1)
     ref = np.real(np.fft.ifft(X * (np.conj(H) if flags & estl_amd.CONV_CORRELATE else H)[None, :], axis=1))
+    flags |= estl_amd.CONV_FULL_LENGTH  # whole-row transforms (not overlap-save windows)
     y = conv_gpu(torch, gpu, x, h, flags)
     check_rows(y, ref, n)
-    if n <= 2 ** 16 and flags == 0:
+    if n <= 2 ** 16 and flags == estl_amd.CONV_FULL_LENGTH:
         check_rows(y, oracle.conv_circular(x.astype(np.float64), h.astype(np.float64)), n)
     # misaligned row: the pair path
     c = estl_amd.Convolution(n, 1, flags)
@@ -168,8 +170,9 @@ def test_random_configurations_match_fp64(gpu):
     (capped at 2^22 values), filters of 1..2n taps, every flag combination -- each
     checked row by row against the convolution theorem in FP64 (numpy's FFT of the
     folded, rotated, normalised filter).  Covers every pipeline: the fused rows kernel,
-    the pass chain with its middle pass, single rows through half-length transforms,
-    and the spectrum plan with and without its first pass."""
+    overlap-save windows, and (whole-row transforms) the pass chain with its middle
+    pass, single rows through half-length transforms, and the spectrum plan with and
+    without its first pass."""
     import torch
     rng = np.random.default_rng(2024)
     for case in range(40):
@@ -177,9 +180,9 @@ def test_random_configurations_match_fp64(gpu):
         n = 1 << L
         rows = int(min(rng.integers(1, 10), max(1, (1 << 22) // n)))
         k = int(rng.integers(1, 2 * n + 1)) if rng.random() < 0.3 else int(rng.integers(1, min(n, 8192) + 1))
-        bits = int(rng.integers(0, 8))
+        bits = int(rng.integers(0, 16))
         flags = ((estl_amd.CONV_CORRELATE if bits & 1 else 0) | (estl_amd.CONV_NORMALIZE_KERNEL if bits & 2 else 0) |
-                 (estl_amd.CONV_CENTER if bits & 4 else 0))
+                 (estl_amd.CONV_CENTER if bits & 4 else 0) | (estl_amd.CONV_FULL_LENGTH if bits & 8 else 0))
         x = oracle.fill_uniform(rows * n, 100 + case).astype(np.float32).reshape(rows, n)
         h = oracle.fill_uniform(k, 200 + case).astype(np.float32)
         if flags & estl_amd.CONV_NORMALIZE_KERNEL:
@@ -348,3 +351,97 @@ def test_conv_fp16_normalised_output_scale(gpu, n):
     ref = oracle.conv_circular(xq, h64 / h64.sum())
     for r in range(rows):
         assert oracle.rel_l2(y[r], ref[r]) <= fp16_tol(n), r
+
+
+def fp64_reference(x, h, n, flags):
+    """The circular convolution (or correlation) of each row of x with the folded,
+    optionally rotated and normalised filter, by the convolution theorem in FP64."""
+    hp = folded(h.astype(np.float64), n)
+    k = len(h)
+    if flags & estl_amd.CONV_CENTER:
+        hp = np.roll(hp, -(((k + 1) // 2) % n))
+    if flags & estl_amd.CONV_NORMALIZE_KERNEL:
+        hp = hp / hp.sum()
+    H = np.fft.fft(hp)
+    if flags & estl_amd.CONV_CORRELATE:
+        H = np.conj(H)
+    return np.real(np.fft.ifft(np.fft.fft(x.astype(np.float64), axis=1) * H[None, :], axis=1))
+
+
+OLS_CASES = [(2 ** 14, 3, 1, 0), (2 ** 14, 2, 17, estl_amd.CONV_CORRELATE), (2 ** 15, 1, 4097, 0),
+             (2 ** 16, 3, 1000, estl_amd.CONV_CENTER), (2 ** 17, 2, 4096, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER),
+             (2 ** 18, 1, 333, estl_amd.CONV_NORMALIZE_KERNEL), (2 ** 20, 2, 4000, 0),
+             (2 ** 22, 1, 2048, estl_amd.CONV_CORRELATE), (2 ** 22, 2, 8, estl_amd.CONV_CENTER)]
+
+
+@pytest.mark.parametrize("n,rows,k,flags", OLS_CASES)
+def test_overlap_save_matches_fp64(gpu, n, rows, k, flags):
+    """Rows longer than 2^13 with filters of at most 4097 taps run overlap-save: windows
+    of 2^13 values (two per complex transform, from any rows), one launch, the outputs
+    of the last window cut at n and the circular wrap read across the row's ends.
+    Checked against the convolution theorem in FP64 for convolution, correlation,
+    centring and normalisation, at the window-size bound (log2 2^13 terms), against the
+    oracle's direct sum at 2^14, and against the whole-row pipeline
+    (ESTL_CONV_FULL_LENGTH) within the two paths' bounds."""
+    import torch
+    x = oracle.fill_uniform(rows * n, 70).astype(np.float32).reshape(rows, n)
+    h = (oracle.fill_uniform(k, 71) + (1.25 if flags & estl_amd.CONV_NORMALIZE_KERNEL else 0.0)).astype(np.float32)
+    ref = fp64_reference(x, h, n, flags)
+    y = conv_gpu(torch, gpu, x, h, flags)
+    for r in range(rows):
+        assert oracle.rel_l2(y[r], ref[r]) <= 2e-6 * 13, (r, oracle.rel_l2(y[r], ref[r]))
+    if n == 2 ** 14 and not flags & (estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER):
+        check_rows(y, oracle.conv_circular(x.astype(np.float64), h.astype(np.float64)), n)
+    yf = conv_gpu(torch, gpu, x, h, flags | estl_amd.CONV_FULL_LENGTH)
+    for r in range(rows):
+        assert oracle.rel_l2(y[r], yf[r]) <= 2 * tol(n)
+
+
+def test_overlap_save_closed_forms(gpu):
+    """Exact answers: a unit impulse filter returns the rows unchanged, a delayed impulse
+    (k taps, the last one 1) rotates them by k - 1 (a correlation by -(k - 1)), and a
+    constant row convolved with any filter is the filter sum everywhere -- across the
+    window boundaries and the circular wrap."""
+    import torch
+    n, rows = 2 ** 16, 3
+    x = oracle.fill_uniform(rows * n, 72).astype(np.float32).reshape(rows, n)
+    one = np.array([1.0], np.float32)
+    assert np.allclose(conv_gpu(torch, gpu, x, one), x, rtol=0, atol=2e-6)
+    k = 3001
+    d = np.zeros(k, np.float32)
+    d[-1] = 1.0
+    assert np.allclose(conv_gpu(torch, gpu, x, d), np.roll(x, k - 1, axis=1), rtol=0, atol=2e-6)
+    assert np.allclose(conv_gpu(torch, gpu, x, d, estl_amd.CONV_CORRELATE), np.roll(x, -(k - 1), axis=1), rtol=0,
+                       atol=2e-6)
+    c = np.full((1, n), 0.5, np.float32)
+    h = oracle.fill_uniform(777, 73).astype(np.float32)
+    y = conv_gpu(torch, gpu, c, h)
+    assert np.allclose(y, 0.5 * h.astype(np.float64).sum(), rtol=0, atol=1e-4)
+
+
+@pytest.mark.parametrize("n,rows,k,flags", [(2 ** 15, 3, 500, 0), (2 ** 20, 2, 4097, estl_amd.CONV_CORRELATE),
+                                            (2 ** 17, 1, 64, estl_amd.CONV_CENTER | estl_amd.CONV_NORMALIZE_KERNEL)])
+def test_overlap_save_fp16_storage(gpu, n, rows, k, flags):
+    """FP16 storage through overlap-save windows: binary16 rows and per-row scales in,
+    binary16 rows in [-1, 1] and scales (input scale x L) out, within the FP16 bound of
+    the pack and the one store against the FP64 convolution theorem."""
+    import torch
+    x = oracle.fill_uniform(rows * n, 74).astype(np.float32).reshape(rows, n)
+    h = (oracle.fill_uniform(k, 75) + (1.25 if flags & estl_amd.CONV_NORMALIZE_KERNEL else 0.0)).astype(np.float32)
+    dx = torch.from_numpy(x.reshape(-1)).to(gpu)
+    x16 = torch.empty(rows * n, dtype=torch.float16, device=gpu)
+    s_in = torch.empty(rows, dtype=torch.float32, device=gpu)
+    estl_amd.pack_fp16(dx, x16, s_in, n // 2, rows)
+    c = estl_amd.Convolution(n, rows, flags, precision=estl_amd.FP16)
+    c.set_filter(torch.from_numpy(h).to(gpu))
+    y16, s_out = torch.empty_like(x16), torch.empty_like(s_in)
+    c.execute_fp16(x16, s_in, y16, s_out)
+    torch.cuda.synchronize()
+    assert y16.abs().max().item() <= 1.0
+    y = (y16.float().reshape(rows, n) * s_out[:, None]).cpu().numpy().astype(np.float64)
+    # the stored input (binary16 x scale), convolved exactly
+    xs = (x16.float().reshape(rows, n) * s_in[:, None]).cpu().numpy()
+    ref = fp64_reference(xs, h, n, flags)
+    for r in range(rows):
+        # one binary16 store of the output relative to its L1 bound, the inputs exact
+        assert oracle.rel_l2(y[r], ref[r]) <= 2.0 ** -11 * 4, (r, oracle.rel_l2(y[r], ref[r]))

@@ -13,6 +13,8 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
+FULL = estl_amd.CONV_FULL_LENGTH
+
 # (kind, n, rows, k, flags): convolution rows of n real values with a k-value filter,
 # or a batched complex FFT of `rows` rows of n (k unused)
 MIX = [
@@ -164,8 +166,9 @@ def test_served_results_match_oracle(gpu):
 def test_admission_over_budget_and_hold_until_release(gpu):
     import torch
     n, rows = 2 ** 16, 4                     # fused-pass convolution: output 1 MiB
-    x, h = make_inputs(torch, gpu, ("conv", n, rows, 64, 0), 11)
-    ref = alone(torch, gpu, ("conv", n, rows, 64, 0), x, h)
+    case = ("conv", n, rows, 64, FULL)       # whole-row transforms: two scratch buffers
+    x, h = make_inputs(torch, gpu, case, 11)
+    ref = alone(torch, gpu, case, x, h)
     # budget: an output plus one request's workspace, not two outputs plus workspace
     out_bytes = 4 * n * rows
     ws = 2 * 8 * n + 256 + 2 * 8 * n * (rows // 2)
@@ -177,18 +180,18 @@ def test_admission_over_budget_and_hold_until_release(gpu):
         srv.wait(rid_big, 30)
     assert e.value.code == estl_amd.ERR_NOMEM
     srv.release(rid_big)
-    r1 = srv.submit_conv(n, rows, x, h)
-    r2 = srv.submit_conv(n, rows, x, h)
+    r1 = srv.submit_conv(n, rows, x, h, flags=FULL)
+    r2 = srv.submit_conv(n, rows, x, h, flags=FULL)
     srv.wait(r1, 60)
     # r1's output still holds the arena: r2 cannot be admitted until it is released
     with pytest.raises(estl_amd.EstlError) as e:
         srv.wait(r2, 0.5)
     assert e.value.code == estl_amd.ERR_BUSY
     assert srv.poll(r2)[0] == estl_amd.REQ_QUEUED
-    assert torch.equal(result(srv, r1, ("conv", n, rows, 64, 0), torch), ref)
+    assert torch.equal(result(srv, r1, case, torch), ref)
     srv.release(r1)
     srv.wait(r2, 60)
-    assert torch.equal(result(srv, r2, ("conv", n, rows, 64, 0), torch), ref)
+    assert torch.equal(result(srv, r2, case, torch), ref)
     srv.release(r2)
     st = srv.stats()
     assert st["rejected"] == 1 and st["completed"] == 2
@@ -332,12 +335,13 @@ def _hold(torch):
     return side
 
 
-@pytest.mark.parametrize("n,flags", [(2 ** 15, 0), (2 ** 15, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER),
+@pytest.mark.parametrize("n,flags", [(2 ** 15, FULL), (2 ** 15, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER | FULL),
                                      (2 ** 12, 0), (2 ** 13, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER),
-                                     (2 ** 19, 0)])
+                                     (2 ** 19, FULL)])
 def test_request_groups_equal_alone_bitwise(gpu, n, flags):
     """Queued convolutions of one length run as one group (one launch per pass, or one
-    fused launch, for all of them); every member's result is bit-identical to the
+    fused launch, for all of them; rows above 2^13 with whole-row transforms,
+    ESTL_CONV_FULL_LENGTH -- overlap-save requests run alone); every member's result is bit-identical to the
     request run alone, with odd and even row counts, distinct inputs and filters
     (inline, then registered), and a request of another length queued in between.  One
     stream (two requests in flight) whose first two requests wait on a busy input
@@ -394,9 +398,10 @@ def test_request_groups_equal_alone_bitwise(gpu, n, flags):
 
 
 @pytest.mark.parametrize("n,flags,registered,short", [
-    (2 ** 16, estl_amd.CONV_CORRELATE, False, False), (2 ** 11, estl_amd.CONV_CENTER, False, False),
-    (2 ** 16, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER, True, False), (2 ** 12, estl_amd.CONV_CENTER, True, False),
-    (2 ** 16, 0, False, True), (2 ** 19, estl_amd.CONV_CORRELATE, False, True),
+    (2 ** 16, estl_amd.CONV_CORRELATE | FULL, False, False), (2 ** 11, estl_amd.CONV_CENTER, False, False),
+    (2 ** 16, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER | FULL, True, False),
+    (2 ** 12, estl_amd.CONV_CENTER, True, False),
+    (2 ** 16, FULL, False, True), (2 ** 19, estl_amd.CONV_CORRELATE | FULL, False, True),
 ])
 def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags, registered, short):
     """FP16-storage requests of one length, queued behind a busy input stream, run as a
@@ -429,7 +434,7 @@ def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags, registered, sho
     srv = estl_amd.Server(1 << 30, streams=1)
     fids = []
     if registered:
-        fflags = (flags & estl_amd.CONV_CENTER) | estl_amd.CONV_FP16
+        fflags = (flags & (estl_amd.CONV_CENTER | FULL)) | estl_amd.CONV_FP16
         fids = [srv.register_filter(n, h, int(h.numel()), fflags) for (_, _, h) in keep]
         for f in fids:
             srv.wait(f, 60)
@@ -458,9 +463,9 @@ def test_fp16_request_groups_equal_object_bitwise(gpu, n, flags, registered, sho
     srv.close()
 
 
-@pytest.mark.parametrize("n,flags,min_grouped", [(2 ** 16, 0, 6),
-                                                 (2 ** 20, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER, 6),
-                                                 (2 ** 14, estl_amd.CONV_CORRELATE, 2)])
+@pytest.mark.parametrize("n,flags,min_grouped", [(2 ** 16, FULL, 6),
+                                                 (2 ** 20, estl_amd.CONV_CORRELATE | estl_amd.CONV_CENTER | FULL, 6),
+                                                 (2 ** 14, estl_amd.CONV_CORRELATE | FULL, 2)])
 def test_single_row_groups_equal_alone_bitwise(gpu, n, flags, min_grouped):
     """Single FP32 rows run through transforms of n/2 values; queued single rows of one
     length group with one another (each member's row read and written as one complex
@@ -514,11 +519,11 @@ def test_random_stream_equals_alone_bitwise(gpu):
     filters -- submitted at once (the first two held on a busy input stream, so the rest
     queue, group and run beside one another on two streams).  Every output (and FP16
     row scales) is bit-identical to the same convolution run alone through the object
-    API, whatever group, path (pairs, single rows at half length, fused rows) or
-    neighbours it had."""
+    API, whatever group, path (pairs, single rows at half length, fused rows,
+    overlap-save windows) or neighbours it had."""
     import torch
     rng = np.random.default_rng(99)
-    flag_bits = (estl_amd.CONV_CORRELATE, estl_amd.CONV_CENTER, estl_amd.CONV_NORMALIZE_KERNEL)
+    flag_bits = (estl_amd.CONV_CORRELATE, estl_amd.CONV_CENTER, estl_amd.CONV_NORMALIZE_KERNEL, FULL)
     cases = []
     for i in range(60):
         n = 1 << int(rng.integers(10, 21))
@@ -601,7 +606,7 @@ def test_request_groups_at_member_limit(gpu):
     rng = np.random.default_rng(77)
     rows_list = [int(r) for r in rng.integers(1, 9, size=count)]
     ks = [int(k) for k in rng.integers(1, 2000, size=count)]
-    cases = [("conv", n, r, k, estl_amd.CONV_CORRELATE if i % 3 == 0 else 0)
+    cases = [("conv", n, r, k, (estl_amd.CONV_CORRELATE if i % 3 == 0 else 0) | FULL)
              for i, (r, k) in enumerate(zip(rows_list, ks))]
     inputs = [make_inputs(torch, gpu, c, 700 + i) for i, c in enumerate(cases)]
     refs = [alone(torch, gpu, c, x, h) for c, (x, h) in zip(cases, inputs)]
