@@ -73,14 +73,15 @@ def percentile(xs, p):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-def load_traffic():
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter
-    summary (profiles/traffic_k_fft_rows_4096x16384.json), if present."""
-    path = os.path.join(ROOT, "profiles", "traffic_k_fft_rows_4096x16384.json")
+def load_traffic(leg: str = "rows"):
+    """Calibrated HBM traffic of a leg from the committed rocprofv3 counter summaries
+    (profiles/traffic.json, written from `bench.py --counters LEG` runs under
+    `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`), if present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("traffic_bytes_per_launch")
+        return json.load(f).get(leg)
 
 
 def cpu_baseline(rows: int, reps: int):
@@ -181,7 +182,8 @@ def bench_uneven(torch, estl_amd, stream, reps: int = 10):
             "batched_gbs": round(bytes_ / t_batched / 1e9, 1), "launches_batched": 1,
             "per_call_ms": round(t_percall * 1e3, 2), "per_call_gbs": round(bytes_ / t_percall / 1e9, 2),
             "launches_per_call": launches, "batched_over_per_call": round(t_percall / t_batched, 1),
-            "tone_rel_l2_max": worst, "work_items": ub.info["items"], "workgroups": ub.info["workgroups"]}
+            "tone_rel_l2_max": worst, "work_items": ub.info["items"], "workgroups": ub.info["workgroups"],
+            "traffic": load_traffic("uneven")}
 
 
 def bench_large(torch, estl_amd, stream, reps: int = 10, check: bool = True):
@@ -234,6 +236,7 @@ def bench_large(torch, estl_amd, stream, reps: int = 10, check: bool = True):
                            "passes": [[q["radix"], q["tile"]] for q in p16.describe().get("passes", [])],
                            "rel_l2_vs_fp64": oracle.rel_l2(y.cpu().numpy(), ref) if check else None}
     out["bound_fp32"] = 1e-6 * 26
+    out["traffic"] = load_traffic("large")
     return out
 
 
@@ -516,7 +519,79 @@ def bench_serve(torch, estl_amd, stream, load: float = 0.7, budget: int = 96 << 
             "served_over_plain_per_transform": round(wall_pt / wall_cap, 2),
             "filter_bank_capacity_req_s": round(count / wall_bank, 1), "filter_bank_registered": len(filters),
             "arena_peak_bytes": st["arena_peak_bytes"], "peak_inflight": st["peak_inflight"],
-            "rejected": st["rejected"], "failed": failed, "alone_equals_mix": same, "fp16_storage": fp16_mode}
+            "rejected": st["rejected"], "failed": failed, "alone_equals_mix": same, "fp16_storage": fp16_mode,
+            "traffic": load_traffic("served")}
+
+
+def counters_leg(leg_name, torch, estl_amd, stream, fwd, x, y):
+    """The launches one `rocprofv3 --pmc` run measures: set-up first (plans, tuning,
+    inputs), then a k_fill_uniform launch that marks where the measured dispatches
+    start, then the leg's launches.  Returns what the counter summary needs (the
+    algorithmic bytes of the measured launches)."""
+    marker = torch.empty(256, dtype=torch.float32, device="cuda:0")
+    if leg_name == "rows":
+        estl_amd.fill_uniform(marker, 0, stream=stream)
+        for _ in range(3):
+            fwd.execute(x, y, stream)
+        stream.synchronize()
+        return {"launches": 3, "algorithmic_bytes_per_launch": 16 * N * BATCH}
+    if leg_name == "large":
+        n = 1 << 26
+        xl = torch.empty(n, dtype=torch.complex64, device="cuda:0")
+        yl = torch.empty_like(xl)
+        estl_amd.fill_uniform(xl, 2, stream=stream)
+        p = estl_amd.Plan(n, 1, estl_amd.FORWARD, estl_amd.FP32, estl_amd.FLAG_AUTOTUNE)
+        x16 = torch.empty(2 * n, dtype=torch.float16, device="cuda:0")
+        y16 = torch.empty_like(x16)
+        s_in = torch.empty(1, dtype=torch.float32, device="cuda:0")
+        s_out = torch.empty_like(s_in)
+        estl_amd.pack_fp16(xl, x16, s_in, n, 1, stream)
+        p16 = estl_amd.Plan(n, 1, estl_amd.FORWARD, estl_amd.FP16, estl_amd.FLAG_AUTOTUNE)
+        stream.synchronize()
+        estl_amd.fill_uniform(marker, 0, stream=stream)
+        for _ in range(3):
+            p.execute(xl, yl, stream)
+        for _ in range(3):
+            p16.execute_fp16(x16, s_in, y16, s_out, stream)
+        stream.synchronize()
+        return {"fp32_passes": [[q["radix"], q["tile"]] for q in p.describe()["passes"]],
+                "fp16_passes": [[q["radix"], q["tile"]] for q in p16.describe()["passes"]],
+                "executes": 3, "fp32_algorithmic_bytes_per_execute": 16 * n,
+                "fp16_algorithmic_bytes_per_execute": 8 * n}
+    if leg_name == "uneven":
+        import estl_amd.workloads as wl
+        b = wl.uneven_batch()
+        xu = torch.empty(b.total, dtype=torch.complex64, device="cuda:0")
+        yu = torch.empty_like(xu)
+        with torch.cuda.stream(stream):
+            wl.fill_uneven(b, xu, torch)
+        ub = estl_amd.UnevenBatch(b.sizes.tolist())
+        ub.execute(xu, yu, stream)
+        stream.synchronize()
+        estl_amd.fill_uniform(marker, 0, stream=stream)
+        for _ in range(3):
+            ub.execute(xu, yu, stream)
+        stream.synchronize()
+        return {"launches": 3, "algorithmic_bytes_per_launch": 16 * int(b.total)}
+    # served: one capacity run of the stream (every request with its own filter)
+    import estl_amd.workloads as wl
+    reqs = wl.conv_requests()
+    count = len(reqs.n)
+    maxv = int(reqs.values().max())
+    xp = torch.empty(maxv, dtype=torch.float32, device="cuda:0")
+    hp = torch.empty(8192, dtype=torch.float32, device="cuda:0")
+    estl_amd.fill_uniform(xp, 4, stream=stream)
+    estl_amd.fill_uniform(hp, 4, offset=maxv, stream=stream)
+    stream.synchronize()
+    srv = estl_amd.Server(96 << 30, streams=8)
+    torch.cuda.synchronize()
+    estl_amd.fill_uniform(marker, 0)
+    torch.cuda.synchronize()
+    wall, _ = _drive(srv, reqs, xp, hp, [0.0] * count)
+    st = srv.stats()
+    srv.close()
+    return {"requests": count, "completed": st["completed"], "groups": st["groups"],
+            "algorithmic_bytes": float(8 * reqs.values().sum() + 4 * reqs.k.sum())}
 
 
 def main() -> int:
@@ -532,9 +607,14 @@ def main() -> int:
     ap.add_argument("--percall-rows", type=int, default=BATCH)
     ap.add_argument("--graph", action="store_true",
                     help="replay the step's two launches as one HIP graph (measured: no gain over eager launches)")
-    ap.add_argument("--counters", action="store_true",
-                    help="counter-collection mode for rocprofv3 --pmc: 3 forward launches of the "
-                         "batched plan and 3 plain 512 MiB device copies (the calibration kernel), nothing else")
+    ap.add_argument("--counters", nargs="?", const="rows", default=None,
+                    choices=["rows", "large", "uneven", "served"],
+                    help="counter-collection mode for rocprofv3 --pmc: after a marker launch "
+                         "(k_fill_uniform), the leg's launches -- rows: 3 forward launches of the batched "
+                         "plan; large: 3 executes each of the tuned FP32 and FP16-storage 2^26 plans; "
+                         "uneven: 3 eager work-queue launches of the 8192-size batch; served: one "
+                         "capacity run of the served stream -- then 3 plain 512 MiB device copies "
+                         "(the calibration kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -565,13 +645,12 @@ def main() -> int:
     plan_desc = fwd.describe()
 
     if args.counters:
+        info = counters_leg(args.counters, torch, estl_amd, stream, fwd, x, y)
         with torch.cuda.stream(stream):
-            for _ in range(3):
-                fwd.execute(x, y, stream)
             for _ in range(3):
                 z.copy_(x)
         stream.synchronize()
-        print(json.dumps({"counters_mode": True, "launch_bytes": 16 * N * BATCH,
+        print(json.dumps({"counters_mode": args.counters, **info,
                           "copy_bytes_read": 8 * N * BATCH, "copy_bytes_written": 8 * N * BATCH}), flush=True)
         return 0
 
@@ -713,7 +792,8 @@ def main() -> int:
         "hbm_frac": round(value / HBM_PEAK_GBS, 4),
         "gflops_5nlog2n": round(2 * 5.0 * N * math.log2(N) * BATCH / (wall / args.steps) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (load_traffic("rows") or {}).get("traffic_bytes_per_launch"),
                      "kernel": "k_fft_rows<12,-1,1> (forward, N=4096)",
                      "launch_ms_mean": round(fwd_mean, 5), "backward_launch_ms_mean": round(bwd_mean, 5),
                      "device_copy_gbs": round(copy_gbs, 1),

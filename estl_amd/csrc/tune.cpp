@@ -4,10 +4,12 @@
 // (FFT_OpenCL_Contiguous.h:55-60), each pass a separate generated kernel.  Here the plan
 // space is (a) the single-launch rows kernel when n <= 2^13 and (b) every split of log2 n
 // into 2..4 global passes of radix 2^4..2^13 near the balanced split, each with its
-// default tile width.  The tuner times each candidate on the device (HIP events, min of
-// a few runs on zero-filled buffers of the real size) and keeps the fastest per
-// (device, log2 n, batch, direction).  Choices persist, one line per key, in the text
-// file named by ESTL_TUNE_CACHE when that variable is set.
+// default tile width, then the tile widths of the three fastest splits and of the
+// fastest split with the fewest passes refined pass by pass.  The tuner times each
+// candidate on the device (HIP events, min of a few runs on zero-filled buffers of the
+// real size) and keeps the fastest per (device, log2 n, batch, direction), preferring
+// fewer passes within 2 %.  Choices persist, one line per key, in the text file named
+// by ESTL_TUNE_CACHE when that variable is set.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -229,6 +231,7 @@ int tune_plan(long n, long batch, int dir, int precision, bool *rows, std::vecto
     std::vector<Choice> cands = candidates(log2n, fp16 ? max_pass_log2_fp16() : kMaxPassLog2);
     Choice best;
     best.us = 1e30;
+    std::vector<Choice> timed;  // pass splits that ran, fastest first after the first stage
     std::string rep = "{\"n\": " + std::to_string(n) + ", \"batch\": " + std::to_string(batch) +
                       ", \"direction\": " + std::to_string(dir) + ", \"candidates\": [";
     bool first = true;
@@ -252,32 +255,52 @@ int tune_plan(long n, long batch, int dir, int precision, bool *rows, std::vecto
         rep += buf;
         first = false;
         if (us < best.us) best = c;
+        if (!c.rows) timed.push_back(c);
     }
-    // second stage: tile widths of the best split, one pass at a time (coordinate
-    // descent over 2^-2 .. 2^+1 times the default width; narrow tiles get the XCD remap)
-    if (best.us < 1e30 && !best.rows) {
-        for (size_t q = 0; q < best.split.size(); ++q) {
-            const int b = best.split[q], d = best.tiles[q];
-            for (int lt : {d - 2, d - 1, d + 1}) {
-                if (lt < 0 || lt > kMaxPassCols || lt + b > kMaxPassTileLog2 || lt > log2n - b) continue;
-                Choice c = best;
-                c.tiles[q] = lt;
-                double us = 0;
-                if (time_candidate(c, n, batch, dir, precision, in, out, scales, st, e0, e1, &us)) continue;
-                char buf[256];
-                std::string s = "[";
-                for (size_t k = 0; k < c.split.size(); ++k) s += (k ? ", " : "") + std::to_string(c.split[k]);
-                s += "], \"tiles\": [";
-                for (size_t k = 0; k < c.tiles.size(); ++k) s += (k ? ", " : "") + std::to_string(c.tiles[k]);
-                s += "]";
-                snprintf(buf, sizeof buf, ", {\"plan\": %s, \"us\": %.2f}", s.c_str(), us);
-                rep += buf;
-                if (us < best.us) {
-                    c.us = us;
-                    best = c;
+    std::sort(timed.begin(), timed.end(), [](const Choice &a, const Choice &b) { return a.us < b.us; });
+    // second stage: tile widths, one pass at a time (coordinate descent over 2^-2 ..
+    // 2^+1 times the default width; narrow tiles get the XCD remap), for the three
+    // fastest splits and the fastest split of the fewest passes, so a split whose
+    // default tiles are poor still competes; a plan of fewer passes (fewer HBM round
+    // trips) is kept when it is within 2 % of the fastest
+    if (best.us < 1e30) {
+        std::vector<Choice> seeds;
+        for (const Choice &c : timed) {
+            if ((int)seeds.size() < 3) seeds.push_back(c);
+        }
+        size_t fewest = 0;
+        for (size_t i = 1; i < timed.size(); ++i)
+            if (timed[i].split.size() < timed[fewest].split.size()) fewest = i;
+        if (!timed.empty() && fewest >= seeds.size()) seeds.push_back(timed[fewest]);
+        std::vector<Choice> refined;
+        for (Choice cur : seeds) {
+            for (size_t q = 0; q < cur.split.size(); ++q) {
+                const int b = cur.split[q], d = cur.tiles[q];
+                for (int lt : {d - 2, d - 1, d + 1}) {
+                    if (lt < 0 || lt > kMaxPassCols || lt + b > kMaxPassTileLog2 || lt > log2n - b) continue;
+                    Choice c = cur;
+                    c.tiles[q] = lt;
+                    double us = 0;
+                    if (time_candidate(c, n, batch, dir, precision, in, out, scales, st, e0, e1, &us)) continue;
+                    char buf[256];
+                    std::string s = "[";
+                    for (size_t k = 0; k < c.split.size(); ++k) s += (k ? ", " : "") + std::to_string(c.split[k]);
+                    s += "], \"tiles\": [";
+                    for (size_t k = 0; k < c.tiles.size(); ++k) s += (k ? ", " : "") + std::to_string(c.tiles[k]);
+                    s += "]";
+                    snprintf(buf, sizeof buf, ", {\"plan\": %s, \"us\": %.2f}", s.c_str(), us);
+                    rep += buf;
+                    if (us < cur.us) {
+                        c.us = us;
+                        cur = c;
+                    }
                 }
             }
+            refined.push_back(cur);
+            if (cur.us < best.us) best = cur;
         }
+        for (const Choice &c : refined)
+            if (!best.rows && c.split.size() < best.split.size() && c.us <= 1.02 * best.us) best = c;
     }
     cleanup();
     rep += "]}";
