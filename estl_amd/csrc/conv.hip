@@ -104,7 +104,7 @@ hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift,
         hipLaunchKernelGGL(k_filter_spectrum<L>, dim3(blocks), dim3(BlockGeom<L>::TPT), 0, st, h, k, shift, spec, \
                            tw, normalize ? 1 : 0, group, divide_l1 ? 1 : 0, dc);                                 \
         break;
-        FS(1) FS(2) FS(3) FS(4) FS(5) FS(6) FS(7) FS(8) FS(9) FS(10) FS(11) FS(12) FS(13)
+        FS(1) FS(2) FS(3) FS(4) FS(5) FS(6) FS(7) FS(8) FS(9) FS(10) FS(11) FS(12) FS(13) FS(14)
 #undef FS
         default: return hipErrorInvalidValue;
     }
@@ -401,10 +401,9 @@ static hipError_t launch_conv_ols_t(const void *x, void *y, const OlsArgs &a, co
     return hipGetLastError();
 }
 
-// x, y: rows of n FP32 (fp16 = false) or binary16 reals; spectrum and table of L = 2^13
-hipError_t launch_conv_ols(bool fp16, bool corr, const void *x, void *y, long rows, int log2n, long k, long oshift,
-                           const float2 *spec, const float2 *tw, hipStream_t st) {
-    constexpr int LOG2L = kOlsLog2;
+template <int LOG2L>
+static hipError_t launch_conv_ols_l(bool fp16, bool corr, const void *x, void *y, long rows, int log2n, long k,
+                                    long oshift, const float2 *spec, const float2 *tw, hipStream_t st) {
     const long n = 1L << log2n;
     const long M = (1L << LOG2L) - k + 1;
     if (log2n <= LOG2L || k < 1 || M < 1 || rows < 1) return hipErrorInvalidValue;
@@ -420,6 +419,17 @@ hipError_t launch_conv_ols(bool fp16, bool corr, const void *x, void *y, long ro
                     : launch_conv_ols_t<LOG2L, false, _Float16>(x, y, a, spec, tw, st);
     return corr ? launch_conv_ols_t<LOG2L, true, float>(x, y, a, spec, tw, st)
                 : launch_conv_ols_t<LOG2L, false, float>(x, y, a, spec, tw, st);
+}
+
+// x, y: rows of n FP32 (fp16 = false) or binary16 reals; spectrum and table of
+// L = 2^log2L, log2L = 13 (512-thread workgroups, two per CU) or 14 (1024 threads, one)
+hipError_t launch_conv_ols(bool fp16, bool corr, int log2L, const void *x, void *y, long rows, int log2n, long k,
+                           long oshift, const float2 *spec, const float2 *tw, hipStream_t st) {
+    if (log2L == kOlsLog2)
+        return launch_conv_ols_l<kOlsLog2>(fp16, corr, x, y, rows, log2n, k, oshift, spec, tw, st);
+    if (log2L == kOlsWideLog2)
+        return launch_conv_ols_l<kOlsWideLog2>(fp16, corr, x, y, rows, log2n, k, oshift, spec, tw, st);
+    return hipErrorInvalidValue;
 }
 
 template <int LOG2N, bool CORR>

@@ -20,6 +20,8 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/estl_hip.h"
@@ -45,8 +47,21 @@ hipError_t launch_conv_fp16_filter(const float *h, long k, float2 *spec, long n,
 hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1, long rows, hipStream_t st);
 hipError_t launch_conv_half_merge(float2 *zv, const float2 *spec, long n, bool corr, hipStream_t st,
                                   const ConvGroup *group = nullptr, int members = 0);
-hipError_t launch_conv_ols(bool fp16, bool corr, const void *x, void *y, long rows, int log2n, long k, long oshift,
-                           const float2 *spec, const float2 *tw, hipStream_t st);
+hipError_t launch_conv_ols(bool fp16, bool corr, int log2L, const void *x, void *y, long rows, int log2n, long k,
+                           long oshift, const float2 *spec, const float2 *tw, hipStream_t st);
+
+int device_cus() {
+    static std::mutex mu;
+    static std::map<int, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    return cache[dev] = cus;
+}
 
 // a single FP32 row long enough for global passes (n >= 2^14) runs at half length
 bool conv_half_path(long n, long rows, int precision) {
@@ -133,8 +148,9 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precisi
         c->scratch_bytes = 0;
         return block_table(current_device(), c->log2n, &c->tw_rows);
     }
-    rc = block_table(current_device(), kOlsLog2, &c->tw_ols);
+    rc = block_table(current_device(), kOlsLog2, &c->tw_ols[0]);
     if (rc) return rc;
+    if (c->log2n > kOlsWideLog2 && (rc = block_table(current_device(), kOlsWideLog2, &c->tw_ols[1]))) return rc;
     // forward passes with the default split; the inverse runs the radices in the reverse
     // order, so the last forward pass and the first inverse pass cover the same columns
     // and fuse (with the spectrum multiply) into one middle pass (pass_tile.h, MID)
@@ -195,12 +211,13 @@ int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k
     const long shift = (c.flags & ESTL_CONV_CENTER) ? ((k + 1) / 2) % c.n : 0;
     const bool normalize = (c.flags & ESTL_CONV_NORMALIZE_KERNEL) != 0;
     hipError_t e = hipSuccess;
-    if (conv_ols_path(c.n, k, c.flags)) {
+    if (const int lw = b.ols) {
         // overlap-save: the unrotated filter's spectrum of the window length (the
         // rotation moves the outputs instead), normalised in the same launch; FP16
         // storage keeps (sum |h|, L) after the window's bins
-        constexpr long L = 1L << kOlsLog2;
-        e = launch_filter_spectrum(kOlsLog2, h, k, 0, b.spec, c.tw_ols, normalize, st, nullptr, 0, false, b.dc);
+        const long L = 1L << lw;
+        e = launch_filter_spectrum(lw, h, k, 0, b.spec, c.tw_ols[lw - kOlsLog2], normalize, st, nullptr, 0, false,
+                                   b.dc);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "filter spectrum launch: %s", hipGetErrorString(e));
         if (c.precision == ESTL_PRECISION_FP16) {
             e = launch_conv_fp16_filter(h, k, b.spec, L, b.dc, normalize, st);
@@ -303,13 +320,18 @@ static int conv_half_chain(const ConvShape &c, const float2 *spec, const float *
     return execute_plan_ws(inv, scratch0, reinterpret_cast<float2 *>(y), scratch1, nullptr, st);
 }
 
-// the overlap-save launch of a request whose filter (b.k taps) takes that path
+// the overlap-save launch of a request whose filter (b.k taps, spectrum of 2^b.ols bins)
+// takes that path
 static int conv_run_ols(const ConvShape &c, const ConvBuffers &b, const void *x, void *y, bool fp16, hipStream_t st) {
     const long shift = (c.flags & ESTL_CONV_CENTER) ? ((b.k + 1) / 2) % c.n : 0;
     // the filter rotation of ESTL_CONV_CENTER as an output rotation: a convolution's
     // result moves back by the shift, a correlation's forward
     const long oshift = c.corr ? shift : -shift;
-    hipError_t e = launch_conv_ols(fp16, c.corr, x, y, c.rows, c.log2n, b.k, oshift, b.spec, c.tw_ols, st);
+    const int lw = b.ols;
+    if (lw != kOlsLog2 && !(lw == kOlsWideLog2 && c.tw_ols[1]))
+        return set_error(ESTL_ERR_INVALID, "no overlap-save window of 2^%d values for n=%ld", lw, c.n);
+    hipError_t e = launch_conv_ols(fp16, c.corr, lw, x, y, c.rows, c.log2n, b.k, oshift, b.spec,
+                                   c.tw_ols[lw - kOlsLog2], st);
     if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "k_conv_ols launch: %s", hipGetErrorString(e));
     return ESTL_OK;
 }
@@ -318,7 +340,7 @@ int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y,
     if (!x || !y) return set_error(ESTL_ERR_INVALID, "null buffer");
     if (c.precision != ESTL_PRECISION_FP32)
         return set_error(ESTL_ERR_INVALID, "an FP16-storage convolution runs through estl_conv_execute_fp16");
-    if (conv_ols_path(c.n, b.k, c.flags)) return conv_run_ols(c, b, x, y, false, st);
+    if (b.ols) return conv_run_ols(c, b, x, y, false, st);
     if (c.half && !(reinterpret_cast<uintptr_t>(x) & 7) && !(reinterpret_cast<uintptr_t>(y) & 7))
         return conv_half_chain(c, b.spec, x, y, b.scratch[0], b.scratch[1], st);
     if (c.fused) {
@@ -446,9 +468,9 @@ int conv_run_fp16(const ConvShape &c, const ConvBuffers &b, const void *x16, con
     if (!x16 || !y16 || !s_in || !s_out) return set_error(ESTL_ERR_INVALID, "null buffer");
     if (c.precision != ESTL_PRECISION_FP16) return set_error(ESTL_ERR_INVALID, "not an FP16-storage convolution");
     hipError_t e;
-    if (conv_ols_path(c.n, b.k, c.flags)) {
+    if (b.ols) {
         if (int rc = conv_run_ols(c, b, x16, y16, true, st)) return rc;
-        e = launch_conv_scales(s_in, s_out, b.spec + (1L << kOlsLog2), c.rows, st);
+        e = launch_conv_scales(s_in, s_out, b.spec + (1L << b.ols), c.rows, st);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "output scales: %s", hipGetErrorString(e));
         return ESTL_OK;
     }
@@ -511,11 +533,15 @@ int estl_conv_create(estl_conv_t *conv, long n, long rows, int precision, unsign
 int estl_conv_set_filter(estl_conv_t conv, const float *d_h, long k, void *stream) {
     if (!conv) return set_error(ESTL_ERR_INVALID, "handle is null");
     Conv *c = reinterpret_cast<Conv *>(conv);
-    int rc = conv_filter(c->shape, c->buf, d_h, k, static_cast<hipStream_t>(stream));
-    if (!rc) {
-        c->have_filter = true;
-        c->buf.k = k;
-    }
+    if (!d_h || k < 1) return set_error(ESTL_ERR_INVALID, "filter needs k >= 1 values");  // the old filter stays
+    const ConvShape &s = c->shape;
+    // a launch of its own: the window length by the device's CU count
+    ConvBuffers b = c->buf;
+    b.k = k;
+    b.ols = conv_ols_log2(s.n, s.rows, k, s.flags, device_cus());
+    int rc = conv_filter(s, b, d_h, k, static_cast<hipStream_t>(stream));
+    c->have_filter = rc == ESTL_OK;  // a failed call may have overwritten the spectrum
+    c->buf = b;
     return rc;
 }
 

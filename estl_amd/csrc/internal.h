@@ -114,9 +114,11 @@ constexpr int kMaxPassLog2 = 13;   // largest radix of a global pass
 constexpr int kMaxPassTileLog2 = 14;  // a pass tile holds at most 2^14 values (~139 KiB of LDS)
 constexpr int kMaxPassCols = 6;    // at most 2^6 columns per tile
 constexpr int kOlsLog2 = 13;       // overlap-save convolution windows of 2^13 values
+constexpr int kOlsWideLog2 = 14;   // and of 2^14 values (rows of n >= 2^15)
 #ifndef ESTL_OLS_MAX_TAPS
 #define ESTL_OLS_MAX_TAPS 4097
 #endif
-constexpr long kOlsMaxTaps = ESTL_OLS_MAX_TAPS;  // filters up to this length take overlap-save
+constexpr long kOlsMaxTaps = ESTL_OLS_MAX_TAPS;  // filters up to this length fit 2^13 windows
+constexpr long kOlsWideMaxTaps = 8193;           // and up to this length 2^14 windows
 
 }  // namespace estl

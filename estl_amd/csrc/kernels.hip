@@ -252,7 +252,7 @@ long rows_twiddle_count(int log2n) {
 #define TW_CASE(L) \
     case L: return BlockGeom<L>::twsize;
         TW_CASE(1) TW_CASE(2) TW_CASE(3) TW_CASE(4) TW_CASE(5) TW_CASE(6) TW_CASE(7) TW_CASE(8)
-        TW_CASE(9) TW_CASE(10) TW_CASE(11) TW_CASE(12) TW_CASE(13)
+        TW_CASE(9) TW_CASE(10) TW_CASE(11) TW_CASE(12) TW_CASE(13) TW_CASE(14)
 #undef TW_CASE
         default: return -1;
     }

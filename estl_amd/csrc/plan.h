@@ -110,22 +110,62 @@ struct ConvShape {
     // k_conv_half_merge, taken when the row's input and output are 8-byte aligned
     bool half = false;
     Plan half_fwd, half_inv;
-    const float2 *tw_ols = nullptr;  // block table of the overlap-save windows (n > 2^13)
+    // block tables of the overlap-save windows of 2^13 (n > 2^13) and 2^14 (n > 2^14) values
+    const float2 *tw_ols[2] = {nullptr, nullptr};
 };
 bool conv_half_path(long n, long rows, int precision);
-// Rows longer than an overlap-save window (n > 2^kOlsLog2) convolved with a filter of
-// at most kOlsMaxTaps taps run overlap-save unless ESTL_CONV_FULL_LENGTH asks for
-// whole-row transforms: windows of 2^kOlsLog2 values, one launch, the filter's spectrum
-// of that length (conv.hip k_conv_ols)
-inline bool conv_ols_path(long n, long k, unsigned flags) {
-    return n > (1L << kOlsLog2) && k >= 1 && k <= kOlsMaxTaps && !(flags & ESTL_CONV_FULL_LENGTH);
+// Rows longer than an overlap-save window convolved with a filter of at most
+// kOlsWideMaxTaps taps run overlap-save unless ESTL_CONV_FULL_LENGTH asks for whole-row
+// transforms: one launch (conv.hip k_conv_ols) over windows of L = 2^13 values (K <= 4097
+// taps; 512-thread workgroups, two per CU) or 2^14 values (n >= 2^15, K <= 8193; 1024
+// threads, one per CU), each giving M = L - K + 1 outputs, with the filter's L-point
+// spectrum.  Returns log2 L, or 0 for whole-row transforms.
+//
+// Between the two window lengths (both possible) the choice is a cost model fitted to
+// MI355X timings of the two kernels: a CU finishes two concurrent 2^13 window pairs (four
+// windows) in time T, one 2^14 pair in 0.95 T, and a lone 2^13 pair in 0.57 T.
+//   cus > 0 (a launch that has the device to itself): pairs are dealt to `cus` CUs and
+//           the time is that of the CU with the most, so the rounding of pairs to CUs
+//           decides for small workloads (a few pairs per CU);
+//   cus = 0 (launches that share the device, as served requests do): time per output,
+//           the windows' count and size alone.
+// 2^13 windows read fewer padding values for short filters (M/L near 1 for either length,
+// and the last window of a row less wasted); 2^14 windows read each input value fewer
+// times for long filters (L/M = 1.14 against 1.33 at K = 2048).
+inline int conv_ols_log2(long n, long rows, long k, unsigned flags, int cus) {
+    if (k < 1 || rows < 1 || (flags & ESTL_CONV_FULL_LENGTH)) return 0;
+    const bool narrow = n > (1L << kOlsLog2) && k <= kOlsMaxTaps;
+    const bool wide = n > (1L << kOlsWideLog2) && k <= kOlsWideMaxTaps;
+    if (!narrow || !wide) return narrow ? kOlsLog2 : wide ? kOlsWideLog2 : 0;
+    const long w13 = rows * ((n + (1L << kOlsLog2) - k) / ((1L << kOlsLog2) - k + 1));
+    const long w14 = rows * ((n + (1L << kOlsWideLog2) - k) / ((1L << kOlsWideLog2) - k + 1));
+    double t13, t14;
+    if (cus > 0) {
+        const long m13 = ((w13 + 1) / 2 + cus - 1) / cus, m14 = ((w14 + 1) / 2 + cus - 1) / cus;
+        t13 = (double)(m13 / 2) + 0.57 * (double)(m13 % 2);
+        t14 = 0.95 * (double)m14;
+    } else {
+        t13 = (double)w13 / 4.0;
+        t14 = 0.95 * (double)w14 / 2.0;
+    }
+    return t14 < t13 ? kOlsWideLog2 : kOlsLog2;
 }
+// Both window lengths possible (the choice then depends on the rows of the call): a
+// filter registered once keeps both spectra, the 2^14 one at conv_ols_wide_offset
+// (float2 values) after the 2^13 one (FP16 storage: each followed by its L1 pair).
+inline bool conv_ols_both(long n, long k, unsigned flags) {
+    return !(flags & ESTL_CONV_FULL_LENGTH) && n > (1L << kOlsWideLog2) && k >= 1 && k <= kOlsMaxTaps;
+}
+inline long conv_ols_wide_offset(bool fp16) { return (1L << kOlsLog2) + (fp16 ? 1 : 0); }
+// compute units of the current device (cached per device; conv_c.cpp)
+int device_cus();
 int conv_run_group_half(const Plan &fwd, const Plan &inv, const ConvGroup *d_group, int members, long n, bool corr,
                         float2 *z, float2 *ws, hipStream_t st);
 struct ConvBuffers {
     float2 *spec = nullptr, *dc = nullptr;
     float2 *scratch[2] = {nullptr, nullptr};
     long k = 0;  // taps of the filter whose spectrum `spec` holds (picks the pipeline)
+    int ols = 0;  // log2 of the overlap-save window of that spectrum (conv_ols_log2), 0: whole rows
 };
 int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precision = 0);
 size_t conv_footprint(const ConvShape &c);  // bytes of ConvBuffers

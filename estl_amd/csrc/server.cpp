@@ -139,6 +139,9 @@ struct Request {
     const float *h = nullptr;
     const float *xscale = nullptr;  // FP16 storage (ESTL_CONV_FP16): per-row input scales
     long long filter = 0;  // KIND_CONV_REG: the registration it uses
+    // convolutions: log2 of the overlap-save window, 0 for whole-row transforms
+    // (conv_ols_log2 for the request's rows); registrations: nonzero when overlap-save
+    int ols = 0;
     int users = 0;         // KIND_FILTER: requests queued or running on it
     // arena blocks: [0] output (the spectrum for KIND_FILTER), the rest workspace
     std::vector<std::pair<size_t, size_t>> blocks;
@@ -181,6 +184,7 @@ class Server {
         device_ = device;
         nstreams_ = std::max(1, std::min(nstreams, 16));
         if (hipSetDevice(device) != hipSuccess) return set_error(ESTL_ERR_HIP, "hipSetDevice(%d)", device);
+        cus_ = device_cus();
         int rc = arena_.init((size_t)budget);
         if (rc) return rc;
         for (int i = 0; i < nstreams_; ++i) {
@@ -281,6 +285,12 @@ class Server {
                 return give_back(
                     set_error(ESTL_ERR_INVALID, "filter %lld and the request differ in FP16 storage", r.filter));
         }
+        // the window a call of this many rows takes alone (so a served result is bitwise the
+        // single call's); a registration holds both spectra where both lengths are possible
+        if (r.kind == KIND_CONV || (r.kind == KIND_CONV_REG && filter->ols))
+            r.ols = conv_ols_log2(r.n, r.rows, r.k, r.flags, cus_);
+        else if (r.kind == KIND_FILTER)
+            r.ols = conv_ols_log2(r.n, 1, r.k, r.flags, cus_);
         if (filter) ++filter->users;
         r.id = ++next_id_;
         r.t_submit = Clock::now();
@@ -535,7 +545,7 @@ class Server {
     bool groupable(const Request &r) const {
         if (r.kind != KIND_CONV && r.kind != KIND_CONV_REG) return false;
         if (r.flags & ESTL_CONV_NORMALIZE_KERNEL) return false;
-        if (conv_ols_path(r.n, r.k, r.flags)) return false;  // one overlap-save launch alone
+        if (r.ols) return false;  // one overlap-save launch alone
         if (half_row(r) && (r.n <= (2L << kMaxRowsLog2) || (reinterpret_cast<uintptr_t>(r.in) & 7))) return false;
         if (r.n < 2 || r.rows < 1) return false;
         if (r.kind == KIND_CONV_REG) {
@@ -778,7 +788,7 @@ class Server {
                 r.shape = c;
                 sizes.push_back(spectrum_bytes(r, *c));
                 sizes.push_back(256);
-                if (spec_scratch(*c) && !conv_ols_path(r.n, r.k, r.flags)) sizes.push_back(spec_scratch(*c));
+                if (spec_scratch(*c) && !r.ols) sizes.push_back(spec_scratch(*c));
                 r.alg_bytes = 0;
                 return true;
             }
@@ -789,14 +799,18 @@ class Server {
 
     // bytes of each scratch buffer of a convolution request (0: none)
     static size_t request_scratch(const Request &r, const ConvShape &c) {
-        if (conv_ols_path(r.n, r.k, r.flags)) return 0;
+        if (r.ols) return 0;
         return r.kind == KIND_CONV ? conv_scratch_bytes(c) : c.scratch_bytes;
     }
     // bytes of the filter spectrum an inline filter or a registration computes: n bins,
-    // or an overlap-save window's; FP16 storage keeps one more value (the L1 bound)
+    // or an overlap-save window's (a registration: both windows' where both lengths are
+    // possible); FP16 storage keeps one more value per spectrum (the L1 bound)
     static size_t spectrum_bytes(const Request &r, const ConvShape &c) {
-        if (!conv_ols_path(r.n, r.k, r.flags)) return c.spec_bytes;
-        return sizeof(float2) * (size_t)((1L << kOlsLog2) + ((r.flags & ESTL_CONV_FP16) ? 1 : 0));
+        if (!r.ols) return c.spec_bytes;
+        const long e = (r.flags & ESTL_CONV_FP16) ? 1 : 0;
+        if (r.kind == KIND_FILTER && conv_ols_both(r.n, r.k, r.flags))
+            return sizeof(float2) * (size_t)(conv_ols_wide_offset(e != 0) + (1L << kOlsWideLog2) + e);
+        return sizeof(float2) * (size_t)((1L << r.ols) + e);
     }
 
     static size_t fp16_out_scales_offset(long n, long rows) {
@@ -814,14 +828,24 @@ class Server {
         if (r.kind == KIND_FILTER) {
             ConvShape *c = r.shape;
             ConvBuffers b;
+            b.k = r.k;
+            b.ols = r.ols;
             b.spec = ptr(0);
             b.dc = ptr(1);
             if (r.blocks.size() > 2) b.scratch[0] = ptr(2);
+            if (r.ols && conv_ols_both(r.n, r.k, r.flags)) {
+                // both window lengths: the 2^13 spectrum, then the 2^14 one after it
+                b.ols = kOlsLog2;
+                if (int rc = conv_filter(*c, b, r.h, r.k, st)) return rc;
+                b.ols = kOlsWideLog2;
+                b.spec += conv_ols_wide_offset((r.flags & ESTL_CONV_FP16) != 0);
+            }
             return conv_filter(*c, b, r.h, r.k, st);
         }
         ConvShape *c = r.shape;
         ConvBuffers b;
         b.k = r.k;  // an inline filter's taps, or the registered filter's
+        b.ols = r.ols;
         size_t next = 1;
         const size_t sb = request_scratch(r, *c);
         if (sb) {
@@ -927,6 +951,8 @@ class Server {
             // order makes the spectrum ready); others take turns
             if (filter) {
                 r.spectrum = static_cast<const float2 *>(filter->out);
+                if (r.ols == kOlsWideLog2 && conv_ols_both(filter->n, filter->k, filter->flags))
+                    r.spectrum += conv_ols_wide_offset((filter->flags & ESTL_CONV_FP16) != 0);
                 r.stream = filter->state == DONE ? next_stream_++ % nstreams_ : filter->stream;
             } else {
                 r.stream = next_stream_++ % nstreams_;
@@ -1079,6 +1105,7 @@ class Server {
     }
 
     int device_ = 0, nstreams_ = 1, nlaunchers_ = 1, max_inflight_ = 2;
+    int cus_ = 256;  // compute units (the overlap-save window choice)
     long max_queue_ = 1 << 16;
     Arena arena_;
     std::vector<hipStream_t> streams_;

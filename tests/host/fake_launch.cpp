@@ -254,17 +254,18 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precisi
     return ESTL_OK;
 }
 
+int device_cus() { return 256; }
+
 size_t conv_scratch_bytes(const ConvShape &c) {
     const size_t spec_need = c.spec_plan.kind == Plan::PASSES ? c.spec_plan.scratch_bytes : 0;
     return c.scratch_bytes > spec_need ? c.scratch_bytes : spec_need;
 }
 
-// The overlap-save path (conv_ols_path) keeps a spectrum of the window length, 2^kOlsLog2
-// values (plus the FP16 L1 pair): here the window holds the filter's taps themselves,
+// The overlap-save path (conv_ols_log2) keeps a spectrum of the window length, 2^13 or
+// 2^14 values (plus the FP16 L1 pair): here the window holds the filter's taps themselves,
 // (h_i, 0) zero-padded, and the convolution builds the n-point spectrum from them when it
 // runs -- the same buffer sizes and the same data flow as the library's.
-static void ols_taps(const float *h, long k, float2 *spec) {
-    const long L = 1L << kOlsLog2;
+static void ols_taps(const float *h, long k, long L, float2 *spec) {
     for (long i = 0; i < L; ++i) spec[i] = {i < k ? h[i] : 0.0f, 0.0f};
 }
 static void ols_spectrum(const ConvShape &c, const float2 *taps, long k, std::vector<float2> &spec) {
@@ -282,10 +283,10 @@ int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k
     const bool fp16 = c.precision == ESTL_PRECISION_FP16;
     const long n = c.n;
     float2 *spec = b.spec, *scr = b.scratch[0];
-    if (conv_ols_path(n, k, c.flags)) {
-        const long L = 1L << kOlsLog2;
+    if (b.ols) {
+        const long L = 1L << b.ols;
         return launch(st, "ols filter", [=] {
-            ols_taps(h, k, spec);
+            ols_taps(h, k, L, spec);
             if (fp16) spec[L] = {l1_of(h, k), l1_of(h, k)};
         });
     }
@@ -314,7 +315,7 @@ int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y,
     const long n = c.n, rows = c.rows;
     const bool corr = c.corr;
     const float2 *spec = b.spec;
-    if (conv_ols_path(n, b.k, c.flags)) {
+    if (b.ols) {
         const ConvShape *cp = &c;
         const long k = b.k;
         return launch(st, "conv ols", [=] {
@@ -342,7 +343,7 @@ int conv_run_fp16(const ConvShape &c, const ConvBuffers &b, const void *x16, con
                   hipStream_t st) {
     const long n = c.n, rows = c.rows;
     const float2 *spec = b.spec;
-    const long l1_at = conv_ols_path(n, b.k, c.flags) ? (1L << kOlsLog2) : n;
+    const long l1_at = b.ols ? (1L << b.ols) : n;
     return launch(st, "conv fp16", [=] { conv_rows16(x16, s_in, n, rows, spec[l1_at].y, y16, s_out); });
 }
 
