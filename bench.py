@@ -723,6 +723,23 @@ def main() -> int:
     stream.synchronize()
     copy_ms = sorted(a.elapsed_time(b) for a, b in cev)[5]
     copy_gbs = launch_bytes / (copy_ms * 1e-3) / 1e9
+    # and the library's 16-byte streaming copy (plain and nontemporal loads / stores):
+    # what a read-once / write-once kernel reaches, the achievable HBM rate
+    stream_copy_gbs = {}
+    for nt in (False, True):
+        estl_amd.stream_copy(x, w, nt, stream)
+        sev = []
+        for _ in range(10):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            estl_amd.stream_copy(x, w, nt, stream)
+            b.record(stream)
+            sev.append((a, b))
+        stream.synchronize()
+        sc_ms = sorted(a.elapsed_time(b) for a, b in sev)[5]
+        stream_copy_gbs["nontemporal" if nt else "plain"] = round(launch_bytes / (sc_ms * 1e-3) / 1e9, 1)
+    assert torch.equal(w, x)
+    copy16_gbs = max(stream_copy_gbs.values())
     del w
 
     # per-call form: one launch per transform, sequential, same stream, device-resident
@@ -797,7 +814,9 @@ def main() -> int:
                      "kernel": "k_fft_rows<12,-1,1> (forward, N=4096)",
                      "launch_ms_mean": round(fwd_mean, 5), "backward_launch_ms_mean": round(bwd_mean, 5),
                      "device_copy_gbs": round(copy_gbs, 1),
-                     "over_device_copy": round(achieved / copy_gbs, 3)},
+                     "over_device_copy": round(achieved / copy_gbs, 3),
+                     "stream_copy16_gbs": stream_copy_gbs,
+                     "over_stream_copy16": round(achieved / copy16_gbs, 3)},
         "per_call": {"value": round(percall_gbs, 2), "unit": "GB/s", "rows": prow,
                      "launches": 2 * prow, "batched_over_per_call": round(value / percall_gbs, 1)},
         "latency_us": {"step_p50": round(1e3 * percentile(step_ms, 50), 2),

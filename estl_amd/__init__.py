@@ -79,6 +79,7 @@ def lib():
     _sig(L, "estl_plan_describe", ci, [vp, ctypes.c_char_p, ci])
     _sig(L, "estl_plan_launches", ci, [vp])
     _sig(L, "estl_fill_uniform", ci, [vp, cl, u64, u64, vp])
+    _sig(L, "estl_stream_copy", ci, [vp, vp, ll, ci, vp])
     _sig(L, "estl_plan_profile", ci, [vp, vp, vp, ci, ctypes.c_char_p, ci])
     _sig(L, "estl_plan_kernel_flops", ctypes.c_double, [vp, ci])
     _sig(L, "estl_plan_kernel_ms", ctypes.c_double, [vp, ci])
@@ -485,6 +486,15 @@ def fill_uniform(t, seed: int, offset: int = 0, stream=None) -> None:
     import torch
     count = t.numel() * (2 if t.dtype == torch.complex64 else 1)
     check(lib().estl_fill_uniform(t.data_ptr(), count, seed, offset, _stream_ptr(stream)))
+
+
+def stream_copy(src, dst, nontemporal: bool = True, stream=None) -> None:
+    """Copy device tensor src into dst (same byte size, a multiple of 16) with the
+    library's 16-byte streaming-copy kernel."""
+    nbytes = src.numel() * src.element_size()
+    if dst.numel() * dst.element_size() != nbytes:
+        raise ValueError("stream_copy: src and dst differ in size")
+    check(lib().estl_stream_copy(src.data_ptr(), dst.data_ptr(), nbytes, 1 if nontemporal else 0, _stream_ptr(stream)))
 
 
 def tune(n: int, batch: int = 1, direction: int = FORWARD, precision: int = FP32) -> dict:

@@ -343,8 +343,10 @@ struct OlsArgs {
     int pre;      // first valid position in a window
 };
 
+// Four waves per SIMD (at most 128 VGPRs): two 512-thread workgroups per CU at L = 2^13,
+// one of 1024 threads at 2^14.
 template <int LOG2L, bool CORR, typename R>
-__global__ void __launch_bounds__(BlockGeom<LOG2L>::TPT)
+__global__ void __launch_bounds__(BlockGeom<LOG2L>::TPT) __attribute__((amdgpu_waves_per_eu(4)))
 k_conv_ols(const R *__restrict__ x, R *__restrict__ y, OlsArgs a, const float2 *__restrict__ spec,
            const float2 *__restrict__ tw, float scale) {
     using G = BlockGeom<LOG2L>;
@@ -367,13 +369,13 @@ k_conv_ols(const R *__restrict__ x, R *__restrict__ y, OlsArgs a, const float2 *
             const long i = tl + TPT * e;
             v[e] = make_float2(ld_real(x0 + ((s0 + i) & nmask)), two ? ld_real(x1 + ((s1 + i) & nmask)) : 0.0f);
         }
-        block_fft<LOG2L, -1, kLdsPad>(v, lds, tl, tw, true);
+        block_fft<LOG2L, -1, kLdsPad, false, kTwPowers>(v, lds, tl, tw, true);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const float2 hk = spec[tl + TPT * e];
             v[e] = CORR ? cmulc(v[e], hk) : cmul(v[e], hk);
         }
-        block_fft<LOG2L, +1, kLdsPad>(v, lds, tl, tw, true);
+        block_fft<LOG2L, +1, kLdsPad, false, kTwPowers>(v, lds, tl, tw, true);
         const long n = nmask + 1;
         R *y0 = y + (r0 << a.log2n), *y1 = y + (r1 << a.log2n);
 #pragma unroll
@@ -421,8 +423,9 @@ static hipError_t launch_conv_ols_l(bool fp16, bool corr, const void *x, void *y
                 : launch_conv_ols_t<LOG2L, false, float>(x, y, a, spec, tw, st);
 }
 
-// x, y: rows of n FP32 (fp16 = false) or binary16 reals; spectrum and table of
-// L = 2^log2L, log2L = 13 (512-thread workgroups, two per CU) or 14 (1024 threads, one)
+// x, y: rows of n FP32 (fp16 = false) or binary16 reals; spectrum and power table
+// (block_fft kTwPowers) of L = 2^log2L, log2L = 13 (512-thread workgroups, two per CU)
+// or 14 (1024 threads, one)
 hipError_t launch_conv_ols(bool fp16, bool corr, int log2L, const void *x, void *y, long rows, int log2n, long k,
                            long oshift, const float2 *spec, const float2 *tw, hipStream_t st) {
     if (log2L == kOlsLog2)

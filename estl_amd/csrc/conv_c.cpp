@@ -148,9 +148,11 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precisi
         c->scratch_bytes = 0;
         return block_table(current_device(), c->log2n, &c->tw_rows);
     }
-    rc = block_table(current_device(), kOlsLog2, &c->tw_ols[0]);
-    if (rc) return rc;
-    if (c->log2n > kOlsWideLog2 && (rc = block_table(current_device(), kOlsWideLog2, &c->tw_ols[1]))) return rc;
+    for (int w = 0; w < 2; ++w) {
+        if (c->log2n <= kOlsLog2 + w) break;
+        if ((rc = block_table(current_device(), kOlsLog2 + w, &c->tw_ols[w]))) return rc;
+        if ((rc = power_table(current_device(), kOlsLog2 + w, &c->tw_ols_pow[w]))) return rc;
+    }
     // forward passes with the default split; the inverse runs the radices in the reverse
     // order, so the last forward pass and the first inverse pass cover the same columns
     // and fuse (with the spectrum multiply) into one middle pass (pass_tile.h, MID)
@@ -328,10 +330,10 @@ static int conv_run_ols(const ConvShape &c, const ConvBuffers &b, const void *x,
     // result moves back by the shift, a correlation's forward
     const long oshift = c.corr ? shift : -shift;
     const int lw = b.ols;
-    if (lw != kOlsLog2 && !(lw == kOlsWideLog2 && c.tw_ols[1]))
+    if (lw != kOlsLog2 && !(lw == kOlsWideLog2 && c.tw_ols_pow[1]))
         return set_error(ESTL_ERR_INVALID, "no overlap-save window of 2^%d values for n=%ld", lw, c.n);
     hipError_t e = launch_conv_ols(fp16, c.corr, lw, x, y, c.rows, c.log2n, b.k, oshift, b.spec,
-                                   c.tw_ols[lw - kOlsLog2], st);
+                                   c.tw_ols_pow[lw - kOlsLog2], st);
     if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "k_conv_ols launch: %s", hipGetErrorString(e));
     return ESTL_OK;
 }

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -77,6 +78,36 @@ static hipStream_t table_stream(int device) {
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
     streams[device] = s;
     return s;
+}
+
+// Power table of BlockGeom<log2n> (block_fft's kTwPowers): W_{N_q}^g at powoff(q) + g,
+// g in [0, A_q), per pass q >= 1; FP64 on the device, rounded once.
+int power_table(int device, int log2n, const float2 **out) {
+    auto &c = twcache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto key = std::make_tuple(device, 2, (long)log2n, 0L, 0L);
+    auto it = c.tables.find(key);
+    if (it != c.tables.end()) {
+        *out = it->second;
+        return ESTL_OK;
+    }
+    long count = rows_power_count(log2n);
+    if (count < 0) return set_error(ESTL_ERR_UNSUPPORTED, "no block kernel for log2n=%d", log2n);
+    float2 *d = nullptr;
+    hipStream_t ts = table_stream(device);
+    HIP_TRY(hipMalloc(&d, sizeof(float2) * (count > 0 ? count : 1)));
+    long off = 0;
+    for (int q = 1; q < rows_pass_count(log2n); ++q) {
+        long A;
+        int B;
+        rows_pass_params(log2n, q, &A, &B);
+        HIP_TRY(launch_twiddle(d + off, A, 1, A, 1, A * B, ts));
+        off += A;
+    }
+    HIP_TRY(hipStreamSynchronize(ts));
+    c.tables[key] = d;
+    *out = d;
+    return ESTL_OK;
 }
 
 // Table of BlockGeom<log2n>: W_{N_q}^{g s} at [(s-1) A_q + g] per pass q >= 1.
@@ -508,6 +539,14 @@ int estl_plan_describe(estl_plan_t plan, char *buf, int len) {
 int estl_plan_launches(estl_plan_t plan) {
     if (!plan) return set_error(ESTL_ERR_INVALID, "plan is null");
     return plan_launches(reinterpret_cast<Plan *>(plan));
+}
+
+int estl_stream_copy(const void *d_src, void *d_dst, long long bytes, int nontemporal, void *stream) {
+    if (!d_src || !d_dst || bytes < 0) return set_error(ESTL_ERR_INVALID, "bad copy arguments");
+    if ((bytes & 15) || ((reinterpret_cast<uintptr_t>(d_src) | reinterpret_cast<uintptr_t>(d_dst)) & 15))
+        return set_error(ESTL_ERR_INVALID, "the copy needs 16-byte aligned pointers and a multiple of 16 bytes");
+    HIP_TRY(launch_stream_copy(d_src, d_dst, (long)(bytes / 16), nontemporal != 0, static_cast<hipStream_t>(stream)));
+    return ESTL_OK;
 }
 
 int estl_fill_uniform(void *d_dst, long count, uint64_t seed, uint64_t offset, void *stream) {

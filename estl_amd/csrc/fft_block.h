@@ -42,8 +42,47 @@ struct BlockGeom {
         return o;
     }
     static constexpr long twsize = twoff(P);
+    // the power table (TwPowers): W_{N_q}^g for g in [0, A_q) per pass q >= 1, at
+    // powoff(q)
+    static constexpr long powoff(int q) {
+        long o = 0;
+        for (int p = 1; p < q; ++p) o += A(p);
+        return o;
+    }
+    static constexpr long powsize = powoff(P);
     static constexpr bool uses_lds = TPT > 1;
 };
+
+// Twiddle sources of block_fft's passes q >= 1 (each butterfly of pass q, group g, needs
+// W_{N_q}^{g s} for s in [1, B_q)):
+//   kTwBlock:  every W_{N_q}^{g s} read from the block table (B_q - 1 reads per
+//              butterfly; the table is N values, 128 KiB at N = 2^14);
+//   kTwPowers: one read, w = W_{N_q}^g from the power table (sum of the A_q, 34 KiB at
+//              2^14, mostly cache-resident); w^(2^k) by squaring, and w^s as the product
+//              of the w^(2^k) of the set bits of s -- at most six products deep from the
+//              correctly rounded w (a few ulp), with only the log2 B powers live.  Trades
+//              the long-latency reads of a table too big for L1 for VALU work.
+enum { kTwBlock = 0, kTwPowers = 1 };
+template <int B, int DIR>
+__device__ __forceinline__ void apply_twiddle_powers(float2 (&u)[B], float2 w) {
+    constexpr int LB = __builtin_ctz(B);
+    float2 p[LB > 0 ? LB : 1];
+    p[0] = w;
+    StaticFor<1, LB>::run([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        p[k] = cmul(p[k - 1], p[k - 1]);
+    });
+    StaticFor<1, B>::run([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr int lo = __builtin_ctz(s);
+        float2 t = p[lo];
+        StaticFor<lo + 1, LB>::run([&](auto bc) {
+            constexpr int b = decltype(bc)::value;
+            if constexpr ((s >> b) & 1) t = cmul(t, p[b]);
+        });
+        u[s] = ctw<DIR>(u[s], t);
+    });
+}
 
 // LDS layouts of a sequence's exchange buffer.
 //  kLdsPad: one float2 of padding after every 16 values; addresses stay base + constant,
@@ -68,14 +107,15 @@ struct Lds<kLdsXor> {
 };
 
 // On entry v[i] = x[tl + TPT*i]; on exit v[i] = X[tl + TPT*i] (X = DFT of x, sign DIR).
-// `lds` is this sequence's LDS slice (Lds<LAYOUT>::size(N) float2); `tw` the size-N table.
+// `lds` is this sequence's LDS slice (Lds<LAYOUT>::size(N) float2); `tw` the size-N block
+// table (TWSRC = kTwBlock) or the power table (kTwPowers).
 // All threads of the workgroup must call it together (it holds workgroup barriers).
 // If `lds_busy` is set, the slice is still being read on entry and a barrier is placed
 // before the first LDS write.  HOIST_TW lets the compiler keep the twiddles of every pass
 // in registers across calls made from a loop (fastest when the loop runs one size, as
 // in k_fft_rows); callers that dispatch many sizes in one loop pass false so each
 // size's twiddles are reloaded (from L1/L2) instead of all staying live.
-template <int LOG2N, int DIR, int LAYOUT = kLdsPad, bool HOIST_TW = false>
+template <int LOG2N, int DIR, int LAYOUT = kLdsPad, bool HOIST_TW = false, int TWSRC = kTwBlock>
 __device__ __forceinline__ void block_fft(float2 (&v)[BlockGeom<LOG2N>::E], float2 *lds, int tl,
                                           const float2 *__restrict__ tw, bool lds_busy = false) {
     using G = BlockGeom<LOG2N>;
@@ -107,9 +147,13 @@ __device__ __forceinline__ void block_fft(float2 (&v)[BlockGeom<LOG2N>::E], floa
                 float2 u[B];
 #pragma unroll
                 for (int s = 0; s < B; ++s) u[s] = lds[Lds<LAYOUT>::idx(base + S * s)];
-                const float2 *twq = (HOIST_TW ? tw : opaque(tw)) + G::twoff(q) + g;
+                if constexpr (TWSRC == kTwPowers) {
+                    apply_twiddle_powers<B, DIR>(u, (HOIST_TW ? tw : opaque(tw))[G::powoff(q) + g]);
+                } else {
+                    const float2 *twq = (HOIST_TW ? tw : opaque(tw)) + G::twoff(q) + g;
 #pragma unroll
-                for (int s = 1; s < B; ++s) u[s] = ctw<DIR>(u[s], twq[(s - 1) * Aq]);
+                    for (int s = 1; s < B; ++s) u[s] = ctw<DIR>(u[s], twq[(s - 1) * Aq]);
+                }
                 Dft<B, DIR>::run(u);
 #pragma unroll
                 for (int h = 0; h < B; ++h) v[b + NB * h] = u[h];

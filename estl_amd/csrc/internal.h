@@ -104,8 +104,11 @@ inline hipError_t launch_pass(int log2b, int log2t, int dir, const float2 *in, f
 }
 hipError_t launch_twiddle(float2 *dst, long count, int mode, long A, long mult, long nq, hipStream_t st);
 hipError_t launch_fill_uniform(float *dst, long count, uint64_t seed, uint64_t offset, hipStream_t st);
+// copy n16 blocks of 16 bytes (16-byte aligned), nontemporal or plain loads and stores
+hipError_t launch_stream_copy(const void *src, void *dst, long n16, bool nontemporal, hipStream_t st);
 
 long rows_twiddle_count(int log2n);
+long rows_power_count(int log2n);  // entries of BlockGeom<log2n>'s power table
 void rows_pass_params(int log2n, int q, long *A, int *B);
 int rows_pass_count(int log2n);
 

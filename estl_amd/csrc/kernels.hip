@@ -9,6 +9,7 @@
 //                  (replaces the per-element native_cos/native_sin of
 //                  KernelGenerator.h:48)
 //   k_fill_uniform seeded synthetic input (splitmix64 stream)
+//   k_stream_copy  16-byte streaming copy (the achievable-bandwidth reference)
 #include <map>
 #include <mutex>
 #include <utility>
@@ -45,6 +46,50 @@ __global__ void k_fill_uniform(float *__restrict__ dst, long count, uint64_t key
         uint64_t u = splitmix64(key + offset + (uint64_t)i);
         dst[i] = (float)((double)(u >> 40) * (1.0 / 8388608.0) - 1.0);
     }
+}
+
+// ------------------------------------------------------------------ streaming copy
+// The plainest HBM-bound kernel: each thread moves U 16-byte blocks (a wave's loads and
+// stores are 1 KiB contiguous each), one pass, no reuse, optionally nontemporal (as the
+// transforms' streamed rows are).  Its rate is what a read-once / write-once kernel
+// reaches on the device.  U = 1 measured fastest on MI355X (512 MiB: 6.4 TB/s
+// nontemporal, against 6.1 at U = 4 and 4.3 at U = 8; 8-byte accesses 6.2 TB/s).
+typedef unsigned v4u_t __attribute__((ext_vector_type(4)));
+constexpr int kStreamCopyU = 1;
+template <bool NT>
+__global__ void __launch_bounds__(256) k_stream_copy(const v4u_t *__restrict__ src, v4u_t *__restrict__ dst, long n16) {
+    constexpr int U = kStreamCopyU;
+    const long base = (long)blockIdx.x * (256 * U) + threadIdx.x;
+    v4u_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const long i = base + 256L * u;
+        if (i < n16) v[u] = NT ? __builtin_nontemporal_load(src + i) : src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const long i = base + 256L * u;
+        if (i < n16) {
+            if (NT)
+                __builtin_nontemporal_store(v[u], dst + i);
+            else
+                dst[i] = v[u];
+        }
+    }
+}
+
+hipError_t launch_stream_copy(const void *src, void *dst, long n16, bool nontemporal, hipStream_t st) {
+    if (n16 <= 0) return hipSuccess;
+    const long per_block = 256L * kStreamCopyU;
+    const long blocks = (n16 + per_block - 1) / per_block;
+    if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
+    if (nontemporal)
+        hipLaunchKernelGGL(k_stream_copy<true>, dim3((unsigned)blocks), dim3(256), 0, st,
+                           static_cast<const v4u_t *>(src), static_cast<v4u_t *>(dst), n16);
+    else
+        hipLaunchKernelGGL(k_stream_copy<false>, dim3((unsigned)blocks), dim3(256), 0, st,
+                           static_cast<const v4u_t *>(src), static_cast<v4u_t *>(dst), n16);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ batched rows
@@ -254,6 +299,17 @@ long rows_twiddle_count(int log2n) {
         TW_CASE(1) TW_CASE(2) TW_CASE(3) TW_CASE(4) TW_CASE(5) TW_CASE(6) TW_CASE(7) TW_CASE(8)
         TW_CASE(9) TW_CASE(10) TW_CASE(11) TW_CASE(12) TW_CASE(13) TW_CASE(14)
 #undef TW_CASE
+        default: return -1;
+    }
+}
+
+long rows_power_count(int log2n) {
+    switch (log2n) {
+#define POW_CASE(L) \
+    case L: return BlockGeom<L>::powsize;
+        POW_CASE(1) POW_CASE(2) POW_CASE(3) POW_CASE(4) POW_CASE(5) POW_CASE(6) POW_CASE(7) POW_CASE(8)
+        POW_CASE(9) POW_CASE(10) POW_CASE(11) POW_CASE(12) POW_CASE(13) POW_CASE(14)
+#undef POW_CASE
         default: return -1;
     }
 }

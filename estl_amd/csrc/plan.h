@@ -72,6 +72,7 @@ inline size_t planar_bytes(long count) {
 
 int current_device();
 int block_table(int device, int log2n, const float2 **out);
+int power_table(int device, int log2n, const float2 **out);  // block_fft's kTwPowers table
 int linear_table(int device, long count, long mult, long nq, const float2 **out);
 std::vector<int> default_pass_split(int log2n);
 int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned flags, const std::vector<int> *split,
@@ -110,8 +111,10 @@ struct ConvShape {
     // k_conv_half_merge, taken when the row's input and output are 8-byte aligned
     bool half = false;
     Plan half_fwd, half_inv;
-    // block tables of the overlap-save windows of 2^13 (n > 2^13) and 2^14 (n > 2^14) values
+    // tables of the overlap-save windows of 2^13 (n > 2^13) and 2^14 (n > 2^14) values:
+    // block tables (the filter's window spectrum) and power tables (k_conv_ols)
     const float2 *tw_ols[2] = {nullptr, nullptr};
+    const float2 *tw_ols_pow[2] = {nullptr, nullptr};
 };
 bool conv_half_path(long n, long rows, int precision);
 // Rows longer than an overlap-save window convolved with a filter of at most

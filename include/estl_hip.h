@@ -203,6 +203,12 @@ int estl_plan_launches(estl_plan_t plan);
  * No reference counterpart (the reference seeds rand() with time(NULL), tests/Test.cpp:189). */
 int estl_fill_uniform(void *d_dst, long count, uint64_t seed, uint64_t offset, void *stream);
 
+/* Copy `bytes` (a multiple of 16; both pointers 16-byte aligned) from d_src to d_dst
+ * with 16-byte loads and stores, nontemporal if `nontemporal` is set: a plain read-once /
+ * write-once kernel, the achievable-bandwidth reference the benchmark reports beside
+ * the transforms.  No reference counterpart (the reference reports no bandwidth). */
+int estl_stream_copy(const void *d_src, void *d_dst, long long bytes, int nontemporal, void *stream);
+
 /* ---------------------------------------------------------------- uneven batches
  * One launch computes `count` transforms of different power-of-two sizes (2 .. 2^20),
  * transform i of sizes[i] complex64 values at element offset offsets[i] of the in and
