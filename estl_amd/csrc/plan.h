@@ -125,14 +125,13 @@ bool conv_half_path(long n, long rows, int precision);
 // spectrum.  Returns log2 L, or 0 for whole-row transforms.
 //
 // Between the two window lengths (both possible) the choice is a cost model fitted to
-// MI355X timings of the two kernels: a CU finishes two concurrent 2^13 window pairs (four
-// windows) in time T, one 2^14 pair in 0.95 T, and a lone 2^13 pair in 0.57 T.
-//   cus > 0 (a launch that has the device to itself): pairs are dealt to `cus` CUs and
-//           the time is that of the CU with the most, so the rounding of pairs to CUs
-//           decides for small workloads (a few pairs per CU);
-//   cus = 0 (launches that share the device, as served requests do): time per output,
-//           the windows' count and size alone.
-// 2^13 windows read fewer padding values for short filters (M/L near 1 for either length,
+// MI355X timings of the two kernels (68 shapes of 2^22 to 2^28 values, K = 16..4097; the
+// choice is within 4.5 % of the faster length at each).
+// Window pairs are dealt to the `cus` CUs and the time is that of the CU with the most:
+// with T the time a CU takes for two concurrent 2^13 pairs (four windows), a lone 2^13
+// pair takes 0.57 T and a 2^14 pair (one per CU) 1.08 T -- or 0.87 T when every CU has
+// at most one (a single short round).  cus = 0: the windows' count and size alone.
+// 2^13 windows read fewer padding values for short filters (L/M near 1 for either length,
 // and the last window of a row less wasted); 2^14 windows read each input value fewer
 // times for long filters (L/M = 1.14 against 1.33 at K = 2048).
 inline int conv_ols_log2(long n, long rows, long k, unsigned flags, int cus) {
@@ -146,10 +145,10 @@ inline int conv_ols_log2(long n, long rows, long k, unsigned flags, int cus) {
     if (cus > 0) {
         const long m13 = ((w13 + 1) / 2 + cus - 1) / cus, m14 = ((w14 + 1) / 2 + cus - 1) / cus;
         t13 = (double)(m13 / 2) + 0.57 * (double)(m13 % 2);
-        t14 = 0.95 * (double)m14;
+        t14 = (m14 == 1 ? 0.87 : 1.08) * (double)m14;
     } else {
         t13 = (double)w13 / 4.0;
-        t14 = 0.95 * (double)w14 / 2.0;
+        t14 = 1.08 * (double)w14 / 2.0;
     }
     return t14 < t13 ? kOlsWideLog2 : kOlsLog2;
 }
