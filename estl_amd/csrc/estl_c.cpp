@@ -257,7 +257,7 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
             // tiles narrower than a 128-byte line share lines with their neighbours
             st.args.xcd_remap = (lt < 4) ? 1 : 0;
         }
-        int rc = block_table(p->device, bits[q], &st.args.tw_local);
+        int rc = power_table(p->device, bits[q], &st.args.tw_local);
         if (rc) return rc;
         if (A > 1) {
             const long nq = A * B;

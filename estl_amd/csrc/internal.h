@@ -40,7 +40,7 @@ struct PassArgs {
     int lobits;              // two-level pre-twiddle split: m = (hi << lobits) | lo
     int xcd_remap;           // deal consecutive tiles to the same XCD
     float scale;             // applied to the stored values (1 = none)
-    const float2 *tw_local;  // block table of the radix-B DFT
+    const float2 *tw_local;  // power table of the radix-B DFT (block_fft kTwPowers)
     const float2 *tw_lo;     // W_{A B}^{lo}, lo < 2^lobits
     const float2 *tw_hi;     // W_{A B}^{hi << lobits}
     // I/O modes of the convolution pipeline (0 = plain complex rows in and out):

@@ -231,7 +231,7 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
             v[i] = ctw<DIR>(v[i], w);
         }
     }
-    block_fft<LOG2B, DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, busy);
+    block_fft<LOG2B, DIR, kLdsPad, false, kTwPowers>(v, lds + c * LDSP, tl, a.tw_local, busy);
     if (a.io_out == 2) {
         // re / im to real rows 2 row and 2 row + 1 of rout (FP32 or binary16); in a
         // request group, the member's pair and rows
@@ -271,7 +271,7 @@ __device__ __forceinline__ void pass_tile_finish(ST *__restrict__ out, const Pas
             const float2 hk = spec[j + ((tl + TPT * i) << lcpr)];
             v[i] = a.corr ? cmulc(v[i], hk) : cmul(v[i], hk);
         }
-        block_fft<LOG2B, -DIR, kLdsPad>(v, lds + c * LDSP, tl, a.tw_local, true);
+        block_fft<LOG2B, -DIR, kLdsPad, false, kTwPowers>(v, lds + c * LDSP, tl, a.tw_local, true);
     }
     if (F32 && a.planar_out) {
         const long e0 = (P.row << a.log2n) + j;

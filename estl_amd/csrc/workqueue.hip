@@ -97,7 +97,7 @@ __device__ __forceinline__ void large_item(const UArgs &a, const UItem &it, floa
     p1.log2n = L;
     p1.log2S = L - B1;
     p1.scale = 1.0f;
-    p1.tw_local = a.tw.block[B1];
+    p1.tw_local = a.tw.pow[B1];
 #pragma unroll 1
     for (long tile = 0; tile < (1L << (L - B1 - T1)); ++tile)
         pass_tile<B1, DIR, T1>(a.in + off, a.scratch + off, p1, tile, lds, true);
@@ -109,7 +109,7 @@ __device__ __forceinline__ void large_item(const UArgs &a, const UItem &it, floa
     p2.log2S = 0;
     p2.lobits = a.tw.lobits[L];
     p2.scale = a.scale;
-    p2.tw_local = a.tw.block[B2];
+    p2.tw_local = a.tw.pow[B2];
     p2.tw_lo = a.tw.lo[L];
     p2.tw_hi = a.tw.hi[L];
 #pragma unroll 1
