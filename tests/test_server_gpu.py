@@ -325,6 +325,59 @@ def test_fp16_requests_equal_fp16_object_bitwise(gpu):
     srv.close()
 
 
+@pytest.mark.parametrize("fp16", [False, True])
+def test_registered_filter_serves_both_window_lengths(gpu, fp16):
+    """A filter registered for n = 2^16 with 1000 taps keeps the spectra of both
+    overlap-save window lengths; requests on it of 1, 64 and 1024 rows take the window
+    length a call of their rows takes alone (2^13, 2^14 and 2^13 windows on a 256-CU
+    part), and each result is bit-identical to the single Convolution call, in FP32 and
+    in FP16 storage."""
+    import torch
+    n, k = 2 ** 16, 1000
+    srv = estl_amd.Server(4 << 30, streams=3)
+    h = torch.empty(k, dtype=torch.float32, device=gpu)
+    estl_amd.fill_uniform(h, 95)
+    f = srv.register_filter(n, h, k, estl_amd.CONV_FP16 if fp16 else 0)
+    ids, refs, keep = [], [], []
+    for i, rows in enumerate((1, 64, 1024, 1)):
+        fl = estl_amd.CONV_CORRELATE if i == 3 else 0
+        x = torch.empty(rows * n, dtype=torch.float32, device=gpu)
+        estl_amd.fill_uniform(x, 96 + i)
+        c = estl_amd.Convolution(n, rows, fl, precision=estl_amd.FP16 if fp16 else estl_amd.FP32)
+        c.set_filter(h)
+        if fp16:
+            x16 = torch.empty(rows * n, dtype=torch.float16, device=gpu)
+            s_in = torch.empty(rows, dtype=torch.float32, device=gpu)
+            estl_amd.pack_fp16(x, x16, s_in, n // 2, rows)
+            y16, s_out = torch.empty_like(x16), torch.empty_like(s_in)
+            c.execute_fp16(x16, s_in, y16, s_out)
+            refs.append((y16, s_out))
+            keep.append((x16, s_in))
+            ids.append((rows, srv.submit_conv_filter_fp16(f, rows, x16, s_in, fl)))
+        else:
+            y = torch.empty_like(x)
+            c.execute(x, y)
+            refs.append((y,))
+            keep.append((x,))
+            ids.append((rows, srv.submit_conv_filter(f, rows, x, fl)))
+        torch.cuda.synchronize()
+        c.close()
+    srv.drain(120)
+    for (rows, rid), ref in zip(ids, refs):
+        srv.wait(rid, 60)
+        if fp16:
+            y16, s_out = srv.output_fp16(rid, n, rows)
+            torch.cuda.synchronize()
+            assert torch.equal(y16, ref[0]) and torch.equal(s_out, ref[1]), rows
+        else:
+            y = srv.output(rid, (rows * n,), torch.float32)
+            torch.cuda.synchronize()
+            assert torch.equal(y, ref[0]), rows
+        srv.release(rid)
+    assert srv.stats()["failed"] == 0
+    srv.close()
+
+
 def _hold(torch):
     """A stream whose queue is busy for a while (a bounded device sleep): requests
     submitted with it as their input stream stay in flight until it ends, so the ones
