@@ -354,20 +354,27 @@ k_conv_ols(const R *__restrict__ x, R *__restrict__ y, OlsArgs a, const float2 *
     __shared__ float2 lds[Lds<kLdsPad>::size(G::N)];
     const long windows = a.rows * a.W;
     const long pairs = (windows + 1) / 2;
-    const long nmask = (1L << a.log2n) - 1;
+    // offsets inside a row in 32 bits (n <= 2^30): the row bases are workgroup-uniform,
+    // so loads and stores take a scalar base and a 32-bit lane offset; x[(s + i) mod n]
+    // with s >= -pre wraps correctly in unsigned arithmetic because n divides 2^32
+    const unsigned nm = (unsigned)((1L << a.log2n) - 1);
     for (long pr = blockIdx.x; pr < pairs; pr += gridDim.x) {  // uniform across the workgroup
         const int tl = opaque_lane(threadIdx.x);
         const long w0 = 2 * pr, w1 = w0 + 1;
         const bool two = w1 < windows;
-        const long r0 = w0 / a.W, b0 = w0 - r0 * a.W;
-        const long r1 = two ? w1 / a.W : r0, b1 = two ? w1 - r1 * a.W : b0;
-        const R *x0 = x + (r0 << a.log2n), *x1 = x + (r1 << a.log2n);
-        const long s0 = b0 * a.M - a.pre, s1 = b1 * a.M - a.pre;
+        // rows and windows, held in scalar registers (rows < 2^31, windows per row < 2^17)
+        const int r0 = __builtin_amdgcn_readfirstlane((int)(w0 / a.W));
+        const int b0 = __builtin_amdgcn_readfirstlane((int)(w0 - (long)r0 * a.W));
+        const int r1 = two ? __builtin_amdgcn_readfirstlane((int)(w1 / a.W)) : r0;
+        const int b1 = two ? __builtin_amdgcn_readfirstlane((int)(w1 - (long)r1 * a.W)) : b0;
+        const R *x0 = x + ((long)r0 << a.log2n), *x1 = x + ((long)r1 << a.log2n);
+        const unsigned m0 = (unsigned)b0 * (unsigned)a.M, m1 = (unsigned)b1 * (unsigned)a.M;  // first outputs
+        const unsigned s0 = m0 - (unsigned)a.pre + (unsigned)tl, s1 = m1 - (unsigned)a.pre + (unsigned)tl;
         float2 v[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const long i = tl + TPT * e;
-            v[e] = make_float2(ld_real(x0 + ((s0 + i) & nmask)), two ? ld_real(x1 + ((s1 + i) & nmask)) : 0.0f);
+            v[e] = make_float2(ld_real(x0 + ((s0 + (unsigned)(TPT * e)) & nm)),
+                               two ? ld_real(x1 + ((s1 + (unsigned)(TPT * e)) & nm)) : 0.0f);
         }
         block_fft<LOG2L, -1, kLdsPad, false, kTwPowers>(v, lds, tl, tw, true);
 #pragma unroll
@@ -376,15 +383,15 @@ k_conv_ols(const R *__restrict__ x, R *__restrict__ y, OlsArgs a, const float2 *
             v[e] = CORR ? cmulc(v[e], hk) : cmul(v[e], hk);
         }
         block_fft<LOG2L, +1, kLdsPad, false, kTwPowers>(v, lds, tl, tw, true);
-        const long n = nmask + 1;
-        R *y0 = y + (r0 << a.log2n), *y1 = y + (r1 << a.log2n);
+        const unsigned n = nm + 1u, osh = (unsigned)a.oshift;
+        R *y0 = y + ((long)r0 << a.log2n), *y1 = y + ((long)r1 << a.log2n);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int q = tl + TPT * e - a.pre;  // position in the window's valid outputs
             if (q < 0 || q >= a.M) continue;
-            const long p0 = b0 * a.M + q, p1 = b1 * a.M + q;
-            if (p0 < n) st_real(y0 + ((p0 + a.oshift) & nmask), v[e].x * scale);
-            if (two && p1 < n) st_real(y1 + ((p1 + a.oshift) & nmask), v[e].y * scale);
+            const unsigned p0 = m0 + (unsigned)q, p1 = m1 + (unsigned)q;
+            if (p0 < n) st_real(y0 + ((p0 + osh) & nm), v[e].x * scale);
+            if (two && p1 < n) st_real(y1 + ((p1 + osh) & nm), v[e].y * scale);
         }
     }
 }
