@@ -24,20 +24,20 @@ int main() {
     // measured: the faster window length where the two differ by 5 % or more
     expect(1L << 22, 64, 16, 0, 13);
     expect(1L << 22, 64, 3000, 0, 14);
-    expect(1L << 20, 256, 16, 0, 13);
+    expect(1L << 22, 4, 16, 0, 13);
+    expect(1L << 22, 4, 2048, 0, 13);
+    expect(1L << 22, 4, 4097, 0, 14);
+    expect(1L << 21, 16, 1536, 0, 13);
+    expect(1L << 21, 16, 4097, 0, 14);
     expect(1L << 20, 256, 1024, 0, 13);
     expect(1L << 20, 256, 3000, 0, 14);
-    expect(1L << 16, 4096, 16, 0, 13);
+    expect(1L << 20, 8, 3000, 0, 13);
+    expect(1L << 19, 24, 1024, 0, 13);
+    expect(1L << 18, 64, 2048, 0, 13);
+    expect(1L << 18, 64, 4097, 0, 14);
     expect(1L << 16, 4096, 3000, 0, 14);
-    expect(1L << 15, 8192, 16, 0, 13);
-    expect(1L << 15, 8192, 1024, 0, 13);
-    expect(1L << 18, 16, 16, 0, 14);
-    expect(1L << 16, 64, 1024, 0, 14);
-    expect(1L << 20, 8, 1536, 0, 13);
-    expect(1L << 22, 4, 16, 0, 13);
-    expect(1L << 22, 4, 4097, 0, 14);
-    expect(1L << 19, 24, 2048, 0, 14);
-    expect(1L << 21, 6, 1024, 0, 13);
+    expect(1L << 16, 256, 4097, 0, 14);
+    expect(1L << 15, 8192, 2048, 0, 13);
     // bounds: only one length fits, or neither
     expect(1L << 14, 3, 4097, 0, 13);   // rows of 2^14: no 2^14 windows
     expect(1L << 14, 3, 4098, 0, 0);    // ... and no 2^13 window holds 4098 taps

@@ -327,19 +327,19 @@ def test_fp16_requests_equal_fp16_object_bitwise(gpu):
 
 @pytest.mark.parametrize("fp16", [False, True])
 def test_registered_filter_serves_both_window_lengths(gpu, fp16):
-    """A filter registered for n = 2^16 with 1000 taps keeps the spectra of both
-    overlap-save window lengths; requests on it of 1, 64 and 1024 rows take the window
+    """A filter registered for n = 2^16 with 4000 taps keeps the spectra of both
+    overlap-save window lengths; requests on it of 1, 256 and 64 rows take the window
     length a call of their rows takes alone (2^13, 2^14 and 2^13 windows on a 256-CU
     part), and each result is bit-identical to the single Convolution call, in FP32 and
     in FP16 storage."""
     import torch
-    n, k = 2 ** 16, 1000
+    n, k = 2 ** 16, 4000
     srv = estl_amd.Server(4 << 30, streams=3)
     h = torch.empty(k, dtype=torch.float32, device=gpu)
     estl_amd.fill_uniform(h, 95)
     f = srv.register_filter(n, h, k, estl_amd.CONV_FP16 if fp16 else 0)
     ids, refs, keep = [], [], []
-    for i, rows in enumerate((1, 64, 1024, 1)):
+    for i, rows in enumerate((1, 256, 64, 1)):
         fl = estl_amd.CONV_CORRELATE if i == 3 else 0
         x = torch.empty(rows * n, dtype=torch.float32, device=gpu)
         estl_amd.fill_uniform(x, 96 + i)
