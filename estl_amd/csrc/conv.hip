@@ -328,8 +328,9 @@ static long conv_resident(K kernel, int threads) {
 // through L-point transforms (one launch, one HBM read of each window and one write of
 // each output).  Window w of the batch (row r = w / W, block b = w mod W) holds the L
 // input values x_r[(b M - pre + i) mod n], i < L; its circular L-point convolution with
-// the filter equals the circular n-point one for i in [pre, pre + M): pre = K - 1 for a
-// convolution, 0 for a correlation (the filter then reaches forward), M = L - K + 1.
+// the filter equals the circular n-point one for i in [pre, pre + M) whenever
+// pre >= K - 1 (a convolution; any pre >= 0 for a correlation, whose filter reaches
+// forward) and pre + M <= L - K + 1 + (K - 1 for a convolution, 0 for a correlation).
 // Those outputs are positions p = b M + i - pre of the row's result, stored at
 // (p + oshift) mod n for p < n (oshift undoes the filter rotation of ESTL_CONV_CENTER,
 // which rotates the output instead).  Two windows (any two, of any rows) share one
@@ -414,7 +415,11 @@ template <int LOG2L>
 static hipError_t launch_conv_ols_l(bool fp16, bool corr, const void *x, void *y, long rows, int log2n, long k,
                                     long oshift, const float2 *spec, const float2 *tw, hipStream_t st) {
     const long n = 1L << log2n;
-    const long M = (1L << LOG2L) - k + 1;
+    // windows on 128-byte boundaries (ols_first_valid, ols_outputs): every window's loads,
+    // and its stores when unrotated, are whole cache lines, for at most 63 of the
+    // L - K + 1 outputs a window could give
+    const long pre_a = ols_first_valid(k, corr);
+    const long M = ols_outputs(LOG2L, k, corr);
     if (log2n <= LOG2L || k < 1 || M < 1 || rows < 1) return hipErrorInvalidValue;
     OlsArgs a;
     a.rows = rows;
@@ -422,7 +427,7 @@ static hipError_t launch_conv_ols_l(bool fp16, bool corr, const void *x, void *y
     a.oshift = ((oshift % n) + n) % n;
     a.log2n = log2n;
     a.M = (int)M;
-    a.pre = corr ? 0 : (int)(k - 1);
+    a.pre = (int)pre_a;
     if (fp16)
         return corr ? launch_conv_ols_t<LOG2L, true, _Float16>(x, y, a, spec, tw, st)
                     : launch_conv_ols_t<LOG2L, false, _Float16>(x, y, a, spec, tw, st);

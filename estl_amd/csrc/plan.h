@@ -126,11 +126,11 @@ bool conv_half_path(long n, long rows, int precision);
 //
 // Between the two window lengths (both possible) the choice is a cost model fitted to
 // MI355X timings of the two kernels (68 shapes of 2^22 to 2^28 values, K = 16..4097; the
-// choice is within 3 % of the faster length at each, where the two differ by up to 36 %).
-// Window pairs are dealt to the `cus` CUs and the time is that of the CU with the most:
-// with T the time a CU takes for two concurrent 2^13 pairs (four windows), a lone 2^13
-// pair takes 0.40 T and a 2^14 pair (one per CU) 1.18 T -- or 1.02 T when every CU has
-// at most one (a single short round).  cus = 0: the windows' count and size alone.
+// choice is within 1.5 % of the faster length at each, where the two differ by up to
+// 30 %).  Window pairs are dealt to the `cus` CUs and the time is that of the CU with the
+// most: with T the time a CU takes for two concurrent 2^13 pairs (four windows), a lone
+// 2^13 pair takes 0.35 T and a 2^14 pair (one per CU) 1.10 T -- or 0.90 T when every CU
+// has at most one (a single short round).  cus = 0: the windows' count and size alone.
 // 2^13 windows read fewer padding values for short filters (L/M near 1 for either length,
 // and the last window of a row less wasted); 2^14 windows read each input value fewer
 // times for long filters (L/M = 1.14 against 1.33 at K = 2048).
@@ -139,16 +139,16 @@ inline int conv_ols_log2(long n, long rows, long k, unsigned flags, int cus) {
     const bool narrow = n > (1L << kOlsLog2) && k <= kOlsMaxTaps;
     const bool wide = n > (1L << kOlsWideLog2) && k <= kOlsWideMaxTaps;
     if (!narrow || !wide) return narrow ? kOlsLog2 : wide ? kOlsWideLog2 : 0;
-    const long w13 = rows * ((n + (1L << kOlsLog2) - k) / ((1L << kOlsLog2) - k + 1));
-    const long w14 = rows * ((n + (1L << kOlsWideLog2) - k) / ((1L << kOlsWideLog2) - k + 1));
+    const long o13 = ols_outputs(kOlsLog2, k, false), o14 = ols_outputs(kOlsWideLog2, k, false);  // per window
+    const long w13 = rows * ((n + o13 - 1) / o13), w14 = rows * ((n + o14 - 1) / o14);          // windows
     double t13, t14;
     if (cus > 0) {
         const long m13 = ((w13 + 1) / 2 + cus - 1) / cus, m14 = ((w14 + 1) / 2 + cus - 1) / cus;
-        t13 = (double)(m13 / 2) + 0.40 * (double)(m13 % 2);
-        t14 = (m14 == 1 ? 1.02 : 1.18) * (double)m14;
+        t13 = (double)(m13 / 2) + 0.35 * (double)(m13 % 2);
+        t14 = (m14 == 1 ? 0.90 : 1.10) * (double)m14;
     } else {
         t13 = (double)w13 / 4.0;
-        t14 = 1.18 * (double)w14 / 2.0;
+        t14 = 1.10 * (double)w14 / 2.0;
     }
     return t14 < t13 ? kOlsWideLog2 : kOlsLog2;
 }

@@ -31,7 +31,8 @@ int main() {
     expect(1L << 21, 16, 4097, 0, 14);
     expect(1L << 20, 256, 1024, 0, 13);
     expect(1L << 20, 256, 3000, 0, 14);
-    expect(1L << 20, 8, 3000, 0, 13);
+    expect(1L << 20, 8, 2048, 0, 13);
+    expect(1L << 16, 64, 2048, 0, 14);
     expect(1L << 19, 24, 1024, 0, 13);
     expect(1L << 18, 64, 2048, 0, 13);
     expect(1L << 18, 64, 4097, 0, 14);

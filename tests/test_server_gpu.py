@@ -329,7 +329,7 @@ def test_fp16_requests_equal_fp16_object_bitwise(gpu):
 def test_registered_filter_serves_both_window_lengths(gpu, fp16):
     """A filter registered for n = 2^16 with 4000 taps keeps the spectra of both
     overlap-save window lengths; requests on it of 1, 256 and 64 rows take the window
-    length a call of their rows takes alone (2^13, 2^14 and 2^13 windows on a 256-CU
+    length a call of their rows takes alone (2^13, 2^14 and 2^14 windows on a 256-CU
     part), and each result is bit-identical to the single Convolution call, in FP32 and
     in FP16 storage."""
     import torch
