@@ -418,8 +418,8 @@ static hipError_t launch_conv_ols_l(bool fp16, bool corr, const void *x, void *y
     // windows on 128-byte boundaries (ols_first_valid, ols_outputs): every window's loads,
     // and its stores when unrotated, are whole cache lines, for at most 63 of the
     // L - K + 1 outputs a window could give
-    const long pre_a = ols_first_valid(k, corr);
-    const long M = ols_outputs(LOG2L, k, corr);
+    const long pre_a = ols_first_valid(k, corr, fp16);
+    const long M = ols_outputs(LOG2L, k, corr, fp16);
     if (log2n <= LOG2L || k < 1 || M < 1 || rows < 1) return hipErrorInvalidValue;
     OlsArgs a;
     a.rows = rows;

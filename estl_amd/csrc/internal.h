@@ -123,17 +123,18 @@ constexpr int kOlsWideLog2 = 14;   // and of 2^14 values (rows of n >= 2^15)
 #endif
 constexpr long kOlsMaxTaps = ESTL_OLS_MAX_TAPS;  // filters up to this length fit 2^13 windows
 constexpr long kOlsWideMaxTaps = 8193;           // and up to this length 2^14 windows
-// Overlap-save windows start on 128-byte boundaries (32 values): the first valid
-// position of a window moves from pre (K - 1, or 0 for a correlation) up to the next
-// multiple of 32, and the outputs per window are the multiple of 32 that still fits.
-constexpr long kOlsAlign = 32;
-inline long ols_first_valid(long k, bool corr) {
-    const long pre = corr ? 0 : k - 1;
-    return (pre + kOlsAlign - 1) / kOlsAlign * kOlsAlign;
+// Overlap-save windows start on 128-byte boundaries (32 FP32 values, 64 binary16 ones):
+// the first valid position of a window moves from pre (K - 1, or 0 for a correlation) up
+// to the next multiple of the alignment, and the outputs per window are the multiple of
+// it that still fits.
+inline long ols_align(bool fp16) { return fp16 ? 64 : 32; }
+inline long ols_first_valid(long k, bool corr, bool fp16 = false) {
+    const long pre = corr ? 0 : k - 1, al = ols_align(fp16);
+    return (pre + al - 1) / al * al;
 }
-inline long ols_outputs(int log2L, long k, bool corr) {
-    const long pre = corr ? 0 : k - 1;
-    return ((1L << log2L) - k + 1 - (ols_first_valid(k, corr) - pre)) / kOlsAlign * kOlsAlign;
+inline long ols_outputs(int log2L, long k, bool corr, bool fp16 = false) {
+    const long pre = corr ? 0 : k - 1, al = ols_align(fp16);
+    return ((1L << log2L) - k + 1 - (ols_first_valid(k, corr, fp16) - pre)) / al * al;
 }
 
 }  // namespace estl
