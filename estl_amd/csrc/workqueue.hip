@@ -77,7 +77,7 @@ __device__ __forceinline__ void small_item(const UArgs &a, const UItem &it, floa
     const float2 *src = a.in + off + tl;
 #pragma unroll
     for (int i = 0; i < E; ++i) v[i] = ld_stream(src + TPT * i);
-    block_fft<L, DIR, kLdsPad>(v, lds + w * USmall<L>::slice, tl, a.tw.block[L], true);
+    block_fft<L, DIR, kLdsPad, false, kTwPowers>(v, lds + w * USmall<L>::slice, tl, a.tw.pow[L], true);
     if (active) {
         float2 *dst = a.out + off + tl;
 #pragma unroll
