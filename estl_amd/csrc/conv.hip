@@ -26,6 +26,24 @@
 
 namespace estl {
 
+// sum |h_t| over k filter taps by the first 256 threads of a workgroup in a fixed
+// summation order (thread t adds taps t, t + 256, ...; then a tree over the 256
+// partial sums), so every caller gets the same value; every thread of the workgroup
+// must call it (it holds barriers); the result is valid in every thread.
+__device__ __forceinline__ float filter_l1_256(const float *__restrict__ h, long k, float *part) {
+    if (threadIdx.x < 256) {
+        float acc = 0.0f;
+        for (long i = threadIdx.x; i < k; i += 256) acc += fabsf(h[i]);
+        part[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    return part[0];
+}
+
 // spectrum /= Re spectrum[0] (the kernel sum), read from a copy `dc` taken beforehand
 __global__ void k_spec_normalize(float2 *__restrict__ spec, long n, const float2 *__restrict__ dc) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -57,11 +75,12 @@ template <int LOG2N>
 __global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT)
 k_filter_spectrum(const float *__restrict__ h, long k, long shift, float2 *__restrict__ spec,
                   const float2 *__restrict__ tw, int normalize, const ConvGroup *__restrict__ group, int divide_l1,
-                  float2 *__restrict__ dc) {
+                  float2 *__restrict__ dc, float2 *__restrict__ l1_single) {
     using G = BlockGeom<LOG2N>;
     constexpr int N = G::N, E = G::E, TPT = G::TPT;
     __shared__ float2 lds[G::uses_lds ? Lds<kLdsPad>::size(N) : 1];
     __shared__ float s_dc;
+    __shared__ float s_part[TPT >= 256 ? 256 : 1];
     const int tl = opaque_lane(threadIdx.x);
     if (group) {
         const int q = blockIdx.x;
@@ -89,20 +108,38 @@ k_filter_spectrum(const float *__restrict__ h, long k, long shift, float2 *__res
 #pragma unroll
         for (int i = 0; i < E; ++i) v[i] = make_float2(v[i].x * s, v[i].y * s);
     }
+    if constexpr (TPT >= 256) {
+        if (!group && l1_single) {
+            // FP16 storage, one filter: L = sum |h| (divided by |kernel sum| when
+            // normalised), the spectrum times 1 / L, and (L, sum |h|) after it -- the
+            // values of k_filter_l1 then k_spec_l1, in one launch
+            const float l1 = filter_l1_256(h, k, s_part);
+            float L = l1;
+            if (normalize) L /= fabsf(s_dc);
+            if (!(L > 0.0f)) L = 1.0f;
+            const float s = 1.0f / L;
+#pragma unroll
+            for (int i = 0; i < E; ++i) v[i] = make_float2(v[i].x * s, v[i].y * s);
+            if (tl == 0) *l1_single = make_float2(L, l1);
+        }
+    }
 #pragma unroll
     for (int i = 0; i < E; ++i) spec[tl + TPT * i] = v[i];
 }
 
+// l1_single (one filter, FP16 storage, N >= 2^12): the spectrum divided by its L and
+// (L, sum |h|) stored there, in place of launch_conv_fp16_filter's two launches
 hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift, float2 *spec, const float2 *tw,
                                   bool normalize, hipStream_t st, const ConvGroup *group, int members,
-                                  bool divide_l1, float2 *dc) {
+                                  bool divide_l1, float2 *dc, float2 *l1_single) {
     if (group && (members < 1 || members > kGroupMax || normalize)) return hipErrorInvalidValue;
+    if (l1_single && (group || log2n < 12)) return hipErrorInvalidValue;
     const unsigned blocks = group ? (unsigned)members : 1u;
     switch (log2n) {
 #define FS(L)                                                                                                   \
     case L:                                                                                                     \
         hipLaunchKernelGGL(k_filter_spectrum<L>, dim3(blocks), dim3(BlockGeom<L>::TPT), 0, st, h, k, shift, spec, \
-                           tw, normalize ? 1 : 0, group, divide_l1 ? 1 : 0, dc);                                 \
+                           tw, normalize ? 1 : 0, group, divide_l1 ? 1 : 0, dc, l1_single);                      \
         break;
         FS(1) FS(2) FS(3) FS(4) FS(5) FS(6) FS(7) FS(8) FS(9) FS(10) FS(11) FS(12) FS(13) FS(14)
 #undef FS
@@ -336,6 +373,8 @@ static long conv_resident(K kernel, int threads) {
 // which rotates the output instead).  Two windows (any two, of any rows) share one
 // complex transform, re and im; the spectrum is the filter's L-point spectrum.
 struct OlsArgs {
+    const float *s_in;  // FP16 storage: per-row input scales, and the output scales the
+    float *s_out;       // launch writes (s_in x L); null otherwise
     long rows;    // real rows of n values
     long W;       // windows per row, ceil(n / M)
     long oshift;  // output rotation, in [0, n)
@@ -353,6 +392,12 @@ k_conv_ols(const R *__restrict__ x, R *__restrict__ y, OlsArgs a, const float2 *
     using G = BlockGeom<LOG2L>;
     constexpr int E = G::E, TPT = G::TPT;
     __shared__ float2 lds[Lds<kLdsPad>::size(G::N)];
+    if (a.s_out && blockIdx.x == 0) {
+        // FP16 storage: the output row scales, s_in times L (the value after the
+        // spectrum's L bins), as k_conv_scales computes them
+        const float L = spec[(size_t)1 << LOG2L].x;
+        for (long r = threadIdx.x; r < a.rows; r += TPT) a.s_out[r] = a.s_in[r] * L;
+    }
     const long windows = a.rows * a.W;
     const long pairs = (windows + 1) / 2;
     // offsets inside a row in 32 bits (n <= 2^30): the row bases are workgroup-uniform,
@@ -413,7 +458,8 @@ static hipError_t launch_conv_ols_t(const void *x, void *y, const OlsArgs &a, co
 
 template <int LOG2L>
 static hipError_t launch_conv_ols_l(bool fp16, bool corr, const void *x, void *y, long rows, int log2n, long k,
-                                    long oshift, const float2 *spec, const float2 *tw, hipStream_t st) {
+                                    long oshift, const float2 *spec, const float2 *tw, const float *s_in,
+                                    float *s_out, hipStream_t st) {
     const long n = 1L << log2n;
     // windows on 128-byte boundaries (ols_first_valid, ols_outputs): every window's loads,
     // and its stores when unrotated, are whole cache lines, for at most 63 of the
@@ -421,7 +467,10 @@ static hipError_t launch_conv_ols_l(bool fp16, bool corr, const void *x, void *y
     const long pre_a = ols_first_valid(k, corr, fp16);
     const long M = ols_outputs(LOG2L, k, corr, fp16);
     if (log2n <= LOG2L || k < 1 || M < 1 || rows < 1) return hipErrorInvalidValue;
+    if (fp16 != (s_out != nullptr) || (s_out && !s_in)) return hipErrorInvalidValue;
     OlsArgs a;
+    a.s_in = s_in;
+    a.s_out = s_out;
     a.rows = rows;
     a.W = (n + M - 1) / M;
     a.oshift = ((oshift % n) + n) % n;
@@ -437,13 +486,15 @@ static hipError_t launch_conv_ols_l(bool fp16, bool corr, const void *x, void *y
 
 // x, y: rows of n FP32 (fp16 = false) or binary16 reals; spectrum and power table
 // (block_fft kTwPowers) of L = 2^log2L, log2L = 13 (512-thread workgroups, two per CU)
-// or 14 (1024 threads, one)
+// or 14 (1024 threads, one).  FP16 storage: s_in / s_out the per-row scales (the
+// launch also writes s_out = s_in x L, L stored after the spectrum); null for FP32.
 hipError_t launch_conv_ols(bool fp16, bool corr, int log2L, const void *x, void *y, long rows, int log2n, long k,
-                           long oshift, const float2 *spec, const float2 *tw, hipStream_t st) {
+                           long oshift, const float2 *spec, const float2 *tw, const float *s_in, float *s_out,
+                           hipStream_t st) {
     if (log2L == kOlsLog2)
-        return launch_conv_ols_l<kOlsLog2>(fp16, corr, x, y, rows, log2n, k, oshift, spec, tw, st);
+        return launch_conv_ols_l<kOlsLog2>(fp16, corr, x, y, rows, log2n, k, oshift, spec, tw, s_in, s_out, st);
     if (log2L == kOlsWideLog2)
-        return launch_conv_ols_l<kOlsWideLog2>(fp16, corr, x, y, rows, log2n, k, oshift, spec, tw, st);
+        return launch_conv_ols_l<kOlsWideLog2>(fp16, corr, x, y, rows, log2n, k, oshift, spec, tw, s_in, s_out, st);
     return hipErrorInvalidValue;
 }
 

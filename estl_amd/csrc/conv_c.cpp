@@ -35,7 +35,7 @@ hipError_t launch_conv_rows(int log2n, bool corr, const float *x, float *y, long
                             const float2 *tw, float scale, hipStream_t st, const ConvGroup *group = nullptr);
 hipError_t launch_filter_spectrum(int log2n, const float *h, long k, long shift, float2 *spec, const float2 *tw,
                                   bool normalize, hipStream_t st, const ConvGroup *group = nullptr, int members = 0,
-                                  bool divide_l1 = false, float2 *dc = nullptr);
+                                  bool divide_l1 = false, float2 *dc = nullptr, float2 *l1_single = nullptr);
 hipError_t launch_spec_normalize(float2 *spec, long n, const float2 *dc, hipStream_t st);
 hipError_t launch_conv_rows_fp16(int log2n, bool corr, const void *x, void *y, long rows, const float2 *spec,
                                  const float2 *tw, float scale, hipStream_t st, const ConvGroup *group = nullptr);
@@ -48,7 +48,8 @@ hipError_t launch_conv_scales(const float *s_in, float *s_out, const float2 *l1,
 hipError_t launch_conv_half_merge(float2 *zv, const float2 *spec, long n, bool corr, hipStream_t st,
                                   const ConvGroup *group = nullptr, int members = 0);
 hipError_t launch_conv_ols(bool fp16, bool corr, int log2L, const void *x, void *y, long rows, int log2n, long k,
-                           long oshift, const float2 *spec, const float2 *tw, hipStream_t st);
+                           long oshift, const float2 *spec, const float2 *tw, const float *s_in, float *s_out,
+                           hipStream_t st);
 
 int device_cus() {
     static std::mutex mu;
@@ -216,15 +217,13 @@ int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k
     if (const int lw = b.ols) {
         // overlap-save: the unrotated filter's spectrum of the window length (the
         // rotation moves the outputs instead), normalised in the same launch; FP16
-        // storage keeps (sum |h|, L) after the window's bins
+        // storage divides it by L and keeps (L, sum |h|) after the window's bins, in that
+        // launch too
         const long L = 1L << lw;
+        const bool h16 = c.precision == ESTL_PRECISION_FP16;
         e = launch_filter_spectrum(lw, h, k, 0, b.spec, c.tw_ols[lw - kOlsLog2], normalize, st, nullptr, 0, false,
-                                   b.dc);
+                                   b.dc, h16 ? b.spec + L : nullptr);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "filter spectrum launch: %s", hipGetErrorString(e));
-        if (c.precision == ESTL_PRECISION_FP16) {
-            e = launch_conv_fp16_filter(h, k, b.spec, L, b.dc, normalize, st);
-            if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "FP16 filter scaling: %s", hipGetErrorString(e));
-        }
         return ESTL_OK;
     }
     if (c.spec_plan.kind == Plan::ROWS) {
@@ -324,7 +323,9 @@ static int conv_half_chain(const ConvShape &c, const float2 *spec, const float *
 
 // the overlap-save launch of a request whose filter (b.k taps, spectrum of 2^b.ols bins)
 // takes that path
-static int conv_run_ols(const ConvShape &c, const ConvBuffers &b, const void *x, void *y, bool fp16, hipStream_t st) {
+static int conv_run_ols(const ConvShape &c, const ConvBuffers &b, const void *x, void *y, const float *s_in,
+                        float *s_out, hipStream_t st) {
+    const bool fp16 = s_out != nullptr;
     const long shift = (c.flags & ESTL_CONV_CENTER) ? ((b.k + 1) / 2) % c.n : 0;
     // the filter rotation of ESTL_CONV_CENTER as an output rotation: a convolution's
     // result moves back by the shift, a correlation's forward
@@ -333,7 +334,7 @@ static int conv_run_ols(const ConvShape &c, const ConvBuffers &b, const void *x,
     if (lw != kOlsLog2 && !(lw == kOlsWideLog2 && c.tw_ols_pow[1]))
         return set_error(ESTL_ERR_INVALID, "no overlap-save window of 2^%d values for n=%ld", lw, c.n);
     hipError_t e = launch_conv_ols(fp16, c.corr, lw, x, y, c.rows, c.log2n, b.k, oshift, b.spec,
-                                   c.tw_ols_pow[lw - kOlsLog2], st);
+                                   c.tw_ols_pow[lw - kOlsLog2], s_in, s_out, st);
     if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "k_conv_ols launch: %s", hipGetErrorString(e));
     return ESTL_OK;
 }
@@ -342,7 +343,7 @@ int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y,
     if (!x || !y) return set_error(ESTL_ERR_INVALID, "null buffer");
     if (c.precision != ESTL_PRECISION_FP32)
         return set_error(ESTL_ERR_INVALID, "an FP16-storage convolution runs through estl_conv_execute_fp16");
-    if (b.ols) return conv_run_ols(c, b, x, y, false, st);
+    if (b.ols) return conv_run_ols(c, b, x, y, nullptr, nullptr, st);
     if (c.half && !(reinterpret_cast<uintptr_t>(x) & 7) && !(reinterpret_cast<uintptr_t>(y) & 7))
         return conv_half_chain(c, b.spec, x, y, b.scratch[0], b.scratch[1], st);
     if (c.fused) {
@@ -471,10 +472,7 @@ int conv_run_fp16(const ConvShape &c, const ConvBuffers &b, const void *x16, con
     if (c.precision != ESTL_PRECISION_FP16) return set_error(ESTL_ERR_INVALID, "not an FP16-storage convolution");
     hipError_t e;
     if (b.ols) {
-        if (int rc = conv_run_ols(c, b, x16, y16, true, st)) return rc;
-        e = launch_conv_scales(s_in, s_out, b.spec + (1L << b.ols), c.rows, st);
-        if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "output scales: %s", hipGetErrorString(e));
-        return ESTL_OK;
+        return conv_run_ols(c, b, x16, y16, s_in, s_out, st);  // the output scales in the same launch
     }
     if (c.fused) {
         e = launch_conv_rows_fp16(c.log2n, c.corr, x16, y16, c.rows, b.spec, c.tw_rows, c.scale, st);
