@@ -161,7 +161,7 @@ __device__ __forceinline__ void st_real(_Float16 *p, float v) { *p = (_Float16)v
 // every stored output lies in [-1, 1] (|y| <= sum |h| max |x|); the caller's per-row
 // scales carry the magnitudes).
 template <int LOG2N, int TPW, bool CORR, typename R>
-__global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT * TPW)
+__global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT * TPW) __attribute__((amdgpu_waves_per_eu(4)))
 k_conv_rows(const R *__restrict__ x, R *__restrict__ y, long rows, const float2 *__restrict__ spec,
             const float2 *__restrict__ tw, float scale, const ConvGroup *__restrict__ group) {
     using G = BlockGeom<LOG2N>;
@@ -191,13 +191,13 @@ k_conv_rows(const R *__restrict__ x, R *__restrict__ y, long rows, const float2 
 #pragma unroll
         for (int i = 0; i < E; ++i)
             v[i] = make_float2(ld_real(x1 + TPT * i), two ? ld_real(x1 + N + TPT * i) : 0.0f);
-        block_fft<LOG2N, -1, kLdsPad>(v, lds + w * SLICE, tl, tw, true);
+        block_fft<LOG2N, -1, kLdsPad, false, kTwPowers>(v, lds + w * SLICE, tl, tw, true);
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const float2 hk = spec[tl + TPT * i];
             v[i] = CORR ? cmulc(v[i], hk) : cmul(v[i], hk);
         }
-        block_fft<LOG2N, +1, kLdsPad>(v, lds + w * SLICE, tl, tw, true);
+        block_fft<LOG2N, +1, kLdsPad, false, kTwPowers>(v, lds + w * SLICE, tl, tw, true);
         if (active) {
             R *y1 = y + r1 * N + tl;
 #pragma unroll

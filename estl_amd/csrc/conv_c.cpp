@@ -147,7 +147,7 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precisi
     if (c->log2n <= kMaxRowsLog2) {
         c->fused = true;
         c->scratch_bytes = 0;
-        return block_table(current_device(), c->log2n, &c->tw_rows);
+        return power_table(current_device(), c->log2n, &c->tw_rows);  // k_conv_rows: twiddles by powers
     }
     for (int w = 0; w < 2; ++w) {
         if (c->log2n <= kOlsLog2 + w) break;

@@ -100,7 +100,7 @@ struct ConvShape {
     unsigned flags = 0;
     bool corr = false, fused = false;
     float scale = 1.0f;
-    const float2 *tw_rows = nullptr;
+    const float2 *tw_rows = nullptr;  // power table of the fused rows kernel (n <= 2^13)
     Plan spec_plan;                  // forward FFT of the folded filter (batch 1)
     std::vector<PassStep> fwd, inv;  // multi-pass pipeline (n > 2^13): fwd, mid, inv
     PassStep mid{};                  // last forward + multiply + first inverse pass
