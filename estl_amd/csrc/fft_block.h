@@ -63,6 +63,12 @@ struct BlockGeom {
 //              correctly rounded w (a few ulp), with only the log2 B powers live.  Trades
 //              the long-latency reads of a table too big for L1 for VALU work.
 enum { kTwBlock = 0, kTwPowers = 1 };
+// Twiddle source of the rows kernels (k_fft_rows, _planar, _h): powers below 4096 points,
+// where a workgroup holds several rows and reads the twiddles again for each; the block
+// table from 4096 points (k_fft_rows keeps those twiddles in registers across rows).
+// A rows plan's table follows the same rule (rows_table).
+template <int LOG2N>
+constexpr int kRowsTw = LOG2N < 12 ? kTwPowers : kTwBlock;
 template <int B, int DIR>
 __device__ __forceinline__ void apply_twiddle_powers(float2 (&u)[B], float2 w) {
     constexpr int LB = __builtin_ctz(B);

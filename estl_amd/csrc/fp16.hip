@@ -155,7 +155,7 @@ k_fft_rows_h(const f16x2 *__restrict__ in, f16x2 *__restrict__ out, long batch, 
         const f16x2 *src = in + row * N + tl;
 #pragma unroll
         for (int i = 0; i < E; ++i) v[i] = ld_elem(src + TPT * i);
-        block_fft<LOG2N, DIR, kLdsPad>(v, lds + w * SLICE, tl, tw, busy);
+        block_fft<LOG2N, DIR, kLdsPad, false, kRowsTw<LOG2N>>(v, lds + w * SLICE, tl, tw, busy);
         busy = true;
         if (want < batch) {
             f16x2 *dst = out + row * N + tl;

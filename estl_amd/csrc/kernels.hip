@@ -154,7 +154,7 @@ k_fft_rows(const float2 *__restrict__ in, float2 *__restrict__ out, long batch,
             }
             {
                 const int to = opaque_lane(t);
-                block_fft<LOG2N, DIR, kLdsPad>(v, lds + (to / TPT) * SLICE, to % TPT, tw, busy);
+                block_fft<LOG2N, DIR, kLdsPad, false, kRowsTw<LOG2N>>(v, lds + (to / TPT) * SLICE, to % TPT, tw, busy);
             }
             busy = true;
             if (row < batch) {
@@ -191,7 +191,7 @@ k_fft_rows_planar(const float2 *__restrict__ in, float2 *__restrict__ out, long 
         float2 v[E];
 #pragma unroll
         for (int i = 0; i < E; ++i) v[i] = ld_planar(in, row * N + tl + TPT * i);
-        block_fft<LOG2N, DIR, kLdsPad>(v, lds + w * SLICE, tl, tw, busy);
+        block_fft<LOG2N, DIR, kLdsPad, false, kRowsTw<LOG2N>>(v, lds + w * SLICE, tl, tw, busy);
         busy = true;
         if (want < batch) {
 #pragma unroll

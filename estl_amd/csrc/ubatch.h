@@ -26,8 +26,9 @@ struct UItem {
 
 // Device tables of every size class.
 struct UTables {
-    const float2 *block[kUMaxSmallLog2 + 1];  // BlockGeom tables, log2 0..kUMaxSmallLog2
-    const float2 *pow[kUMaxSmallLog2 + 1];    // power tables of the same (pass radices)
+    // power tables (block_fft kTwPowers) of BlockGeom<log2>, log2 0..kUMaxSmallLog2: the
+    // small transforms and the radices of the large ones' passes
+    const float2 *pow[kUMaxSmallLog2 + 1];
     const float2 *lo[kUMaxLog2 + 1];          // pre-twiddle tables of the second pass
     const float2 *hi[kUMaxLog2 + 1];
     int lobits[kUMaxLog2 + 1];

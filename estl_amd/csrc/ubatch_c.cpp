@@ -138,8 +138,7 @@ static int ubatch_create(UBatch **out, int count, const long *sizes, const long 
     UTables &tw = u->args.tw;
     memset(&tw, 0, sizeof tw);
     for (int L = 1; L <= kUMaxSmallLog2; ++L) {
-        int rc = block_table(u->device, L, &tw.block[L]);
-        if (!rc) rc = power_table(u->device, L, &tw.pow[L]);
+        int rc = power_table(u->device, L, &tw.pow[L]);
         if (rc) {
             ubatch_free(u);
             return rc;
