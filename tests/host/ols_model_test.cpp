@@ -50,6 +50,27 @@ int main() {
     expect(1L << 20, 8, 16, ESTL_CONV_CORRELATE | ESTL_CONV_CENTER, 13);  // other flags do not matter
     expect(1L << 20, 0, 16, 0, 0);      // no rows
     expect(1L << 20, 8, 0, 0, 0);       // no taps
+    // window geometry: the first valid position and the outputs per window are multiples of
+    // the alignment (32 FP32 / 64 binary16 values), the valid outputs stay inside the part
+    // of the window where the L-point circular result equals the n-point one, and at most
+    // 2 x alignment - 1 outputs of L - K + 1 are given up
+    {
+        long bad = 0, checked = 0;
+        for (int lw = 13; lw <= 14; ++lw)
+            for (long k = 1; k <= (lw == 13 ? 4097 : 8193); ++k)
+                for (int corr = 0; corr < 2; ++corr)
+                    for (int h = 0; h < 2; ++h) {
+                        const long L = 1L << lw, al = ols_align(h), pre = corr ? 0 : k - 1;
+                        const long f = ols_first_valid(k, corr, h), m = ols_outputs(lw, k, corr, h);
+                        const long end = corr ? L - k + 1 : L;  // valid window positions [pre, end)
+                        ++checked;
+                        if (f % al || m % al || f < pre || f + m > end || m < 1 || m < (L - k + 1) - (2 * al - 1))
+                            ++bad;
+                    }
+        if (bad) ++g_failures;
+        printf("%s window geometry: %ld of %ld (length, taps, correlation, precision) cases off\n",
+               bad ? "FAIL" : "PASS", bad, checked);
+    }
     // a registration keeps both spectra exactly where both lengths are possible
     const bool both_ok = conv_ols_both(1L << 15, 4097, 0) && !conv_ols_both(1L << 15, 4098, 0) &&
                          !conv_ols_both(1L << 14, 100, 0) && !conv_ols_both(1L << 16, 100, ESTL_CONV_FULL_LENGTH) &&
