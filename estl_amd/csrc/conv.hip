@@ -92,8 +92,8 @@ k_filter_spectrum(const float *__restrict__ h, long k, long shift, float2 *__res
     float2 v[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) v[i] = make_float2(fold_tap(h, k, N, shift, tl + TPT * i), 0.0f);
-    // the table of a rows plan of N points (kRowsTw: powers below 4096, else the block table)
-    block_fft<LOG2N, -1, kLdsPad, false, kRowsTw<LOG2N>>(v, lds, tl, tw, false);
+    // kSpecTw: the power table below 4096 points, the block table from 4096
+    block_fft<LOG2N, -1, kLdsPad, false, kSpecTw<LOG2N>>(v, lds, tl, tw, false);
     if (normalize) {
         if (tl == 0) {
             s_dc = v[0].x;  // bin 0

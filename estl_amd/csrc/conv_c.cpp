@@ -143,6 +143,11 @@ int conv_shape_init(ConvShape *c, long n, long rows, unsigned flags, int precisi
         s0.args.io_in = 3;
     } else if (c->spec_plan.kind != Plan::ROWS) {
         return set_error(ESTL_ERR_UNSUPPORTED, "convolution of n=%ld: no spectrum plan", n);
+    } else {
+        // k_filter_spectrum's table (kSpecTw): the rows plan's power table below 4096
+        // points, the block table from 4096
+        c->tw_spec = c->spec_plan.tw_rows;
+        if (c->log2n >= kSpecBlockTwLog2 && (rc = block_table(current_device(), c->log2n, &c->tw_spec))) return rc;
     }
     if (c->log2n <= kMaxRowsLog2) {
         c->fused = true;
@@ -228,7 +233,7 @@ int conv_filter(const ConvShape &c, const ConvBuffers &b, const float *h, long k
     }
     if (c.spec_plan.kind == Plan::ROWS) {
         // fold, transform and normalise in one launch (the kernel sum to b.dc)
-        e = launch_filter_spectrum(c.log2n, h, k, shift, b.spec, c.spec_plan.tw_rows, normalize, st, nullptr, 0,
+        e = launch_filter_spectrum(c.log2n, h, k, shift, b.spec, c.tw_spec, normalize, st, nullptr, 0,
                                    false, b.dc);
         if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "filter spectrum launch: %s", hipGetErrorString(e));
     } else {
@@ -360,7 +365,7 @@ int conv_run(const ConvShape &c, const ConvBuffers &b, const float *x, float *y,
 int conv_filter_group_rows(const ConvShape &c, const ConvGroup *d_group, int members, bool divide_l1,
                            hipStream_t st) {
     if (!c.fused || c.spec_plan.kind != Plan::ROWS) return set_error(ESTL_ERR_INVALID, "not a fused shape");
-    hipError_t e = launch_filter_spectrum(c.log2n, nullptr, 0, 0, nullptr, c.spec_plan.tw_rows, false, st, d_group,
+    hipError_t e = launch_filter_spectrum(c.log2n, nullptr, 0, 0, nullptr, c.tw_spec, false, st, d_group,
                                           members, divide_l1);
     if (e != hipSuccess) return set_error(ESTL_ERR_HIP, "group filter spectra: %s", hipGetErrorString(e));
     return ESTL_OK;

@@ -223,9 +223,8 @@ int build_plan(Plan *p, long n, long batch, int dir, int precision, unsigned fla
     }
     if (p->log2n <= kMaxRowsLog2 && !(split && !split->empty())) {
         p->kind = Plan::ROWS;
-        // the rows kernels' twiddle source (kRowsTw): powers below 4096 points
-        return p->log2n < 12 ? power_table(p->device, p->log2n, &p->tw_rows)
-                             : block_table(p->device, p->log2n, &p->tw_rows);
+        // the rows kernels' twiddle source (kRowsTw): powers
+        return power_table(p->device, p->log2n, &p->tw_rows);
     }
     p->kind = Plan::PASSES;
     std::vector<int> bits = (split && !split->empty()) ? *split : default_pass_split(p->log2n);

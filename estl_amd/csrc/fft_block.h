@@ -17,6 +17,7 @@
 #pragma once
 
 #include "fft_device.h"
+#include "internal.h"
 
 namespace estl {
 
@@ -63,12 +64,21 @@ struct BlockGeom {
 //              correctly rounded w (a few ulp), with only the log2 B powers live.  Trades
 //              the long-latency reads of a table too big for L1 for VALU work.
 enum { kTwBlock = 0, kTwPowers = 1 };
-// Twiddle source of the rows kernels (k_fft_rows, _planar, _h): powers below 4096 points,
-// where a workgroup holds several rows and reads the twiddles again for each; the block
-// table from 4096 points (k_fft_rows keeps those twiddles in registers across rows).
-// A rows plan's table follows the same rule (rows_table).
+// Twiddle source of the rows kernels (k_fft_rows, _one, _planar, _h): powers at every
+// row length (a rows plan's table is the power table, estl_c.cpp).  With powers the
+// one-row kernel of 4096 and 8192 points needs 68-75 VGPRs; the block table's twiddles
+// kept in registers across rows had needed 167-206 (three waves per SIMD at 4096 points,
+// two at 8192) and measured 6-9 % slower, and read again for each row from L2 they cost
+// more.  The price is a few ulp per twiddle: rel-L2 against FP64 1.2e-7 -> 3.0e-7 at
+// 4096 points (the bound is 1.2e-5).
 template <int LOG2N>
-constexpr int kRowsTw = LOG2N < 12 ? kTwPowers : kTwBlock;
+constexpr int kRowsTw = kTwPowers;
+// Twiddle source of the filter spectrum (k_filter_spectrum): the block table from 4096
+// points.  A spectrum is computed once per request or registration, so its speed hardly
+// matters, and the correctly rounded twiddles keep, e.g., a delayed unit impulse's
+// spectrum -- and the convolution by it -- within 2e-6 of the shifted input.
+template <int LOG2N>
+constexpr int kSpecTw = LOG2N < kSpecBlockTwLog2 ? kTwPowers : kTwBlock;
 template <int B, int DIR>
 __device__ __forceinline__ void apply_twiddle_powers(float2 (&u)[B], float2 w) {
     constexpr int LB = __builtin_ctz(B);

@@ -113,6 +113,7 @@ void rows_pass_params(int log2n, int q, long *A, int *B);
 int rows_pass_count(int log2n);
 
 constexpr int kMaxRowsLog2 = 13;   // largest single-launch row length
+constexpr int kSpecBlockTwLog2 = 12;  // filter spectra of at least 2^12 points read the block table
 constexpr int kMaxPassLog2 = 13;   // largest radix of a global pass
 constexpr int kMaxPassTileLog2 = 14;  // a pass tile holds at most 2^14 values (~139 KiB of LDS)
 constexpr int kMaxPassCols = 6;    // at most 2^6 columns per tile

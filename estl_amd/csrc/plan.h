@@ -102,6 +102,7 @@ struct ConvShape {
     float scale = 1.0f;
     const float2 *tw_rows = nullptr;  // power table of the fused rows kernel (n <= 2^13)
     Plan spec_plan;                  // forward FFT of the folded filter (batch 1)
+    const float2 *tw_spec = nullptr;  // k_filter_spectrum's table when spec_plan is a rows plan
     std::vector<PassStep> fwd, inv;  // multi-pass pipeline (n > 2^13): fwd, mid, inv
     PassStep mid{};                  // last forward + multiply + first inverse pass
     size_t scratch_bytes = 0;        // bytes of each of the two scratch buffers
@@ -112,7 +113,7 @@ struct ConvShape {
     bool half = false;
     Plan half_fwd, half_inv;
     // tables of the overlap-save windows of 2^13 (n > 2^13) and 2^14 (n > 2^14) values:
-    // block tables (the filter's window spectrum) and power tables (k_conv_ols)
+    // block tables (the filter's window spectrum, kSpecTw) and power tables (k_conv_ols)
     const float2 *tw_ols[2] = {nullptr, nullptr};
     const float2 *tw_ols_pow[2] = {nullptr, nullptr};
 };
