@@ -5,7 +5,7 @@ batch 16384 rows (512 MiB per direction), re/im uniform[-1,1) from seed 1, resid
 HBM before the timed region.  Metric: whole-step algorithmic HBM throughput in GB/s
 (16 bytes per complex value per direction: one read + one write), beside:
 
-  * roofline of the dominant kernel (k_fft_rows_one, N=4096): algorithmic bytes per launch
+  * roofline of the dominant kernel (k_fft_rows, N=4096): algorithmic bytes per launch
     / mean launch duration from HIP events on the launch stream, against 8 TB/s;
   * the per-call form (one launch per transform, same kernels, same stream) and the
     batched/per-call throughput ratio;
@@ -811,7 +811,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": (load_traffic("rows") or {}).get("traffic_bytes_per_launch"),
-                     "kernel": "k_fft_rows_one<12,-1> (forward, N=4096)",
+                     "kernel": "k_fft_rows<12,-1,1> (forward, N=4096)",
                      "launch_ms_mean": round(fwd_mean, 5), "backward_launch_ms_mean": round(bwd_mean, 5),
                      "device_copy_gbs": round(copy_gbs, 1),
                      "over_device_copy": round(achieved / copy_gbs, 3),

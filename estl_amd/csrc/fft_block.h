@@ -64,9 +64,9 @@ struct BlockGeom {
 //              correctly rounded w (a few ulp), with only the log2 B powers live.  Trades
 //              the long-latency reads of a table too big for L1 for VALU work.
 enum { kTwBlock = 0, kTwPowers = 1 };
-// Twiddle source of the rows kernels (k_fft_rows, _one, _planar, _h): powers at every
+// Twiddle source of the rows kernels (k_fft_rows, _planar, _h): powers at every
 // row length (a rows plan's table is the power table, estl_c.cpp).  With powers the
-// one-row kernel of 4096 and 8192 points needs 68-75 VGPRs; the block table's twiddles
+// rows kernel of 4096 and 8192 points needs 68-75 VGPRs; the block table's twiddles
 // kept in registers across rows had needed 167-206 (three waves per SIMD at 4096 points,
 // two at 8192) and measured 6-9 % slower, and read again for each row from L2 they cost
 // more.  The price is a few ulp per twiddle: rel-L2 against FP64 1.2e-7 -> 3.0e-7 at

@@ -93,14 +93,18 @@ hipError_t launch_stream_copy(const void *src, void *dst, long n16, bool nontemp
 }
 
 // ------------------------------------------------------------------ batched rows
-// Rows of N <= 2048 points: a group is TPW rows, one per TPT = N/16 threads, in a
-// 256-thread workgroup.  The grid is persistent (sized to the resident workgroup count
-// by the launcher): each workgroup walks groups blockIdx.x, blockIdx.x + gridDim.x, ...
-// and issues the loads of its next group before running the current one through LDS,
-// so HBM reads stay in flight during the register/LDS passes.  Input and output are
-// touched once, so both use nontemporal (streaming) accesses.  Rows past the batch end
-// (a partial last group) are computed on whatever the clamped load returned and never
-// stored.
+// Rows of N <= 2^13 points: a group of TPW = max(1, 256 / TPT) rows, one per TPT = N/16
+// threads (16 values per thread), per workgroup, and one workgroup per group -- the
+// whole transform in registers and padded LDS, one HBM read and one HBM write per
+// value, nontemporal loads and stores.  A partial last group computes its missing rows
+// on a copy of the last row and stores nothing for them.  With twiddles by powers
+// (kRowsTw) the kernel holds 68 VGPRs at 4096 points and 75 at 8192, so LDS alone
+// bounds residency (four 34 KiB groups per CU; two 8192-point rows).  Measured against
+// the earlier persistent grid (four resident rounds of workgroups, the next group
+// prefetched into registers; at 4096 and 8192 points the block table's twiddles held in
+// registers across rows, 167 / 206 VGPRs): 6.5 % faster at 4096 x 16384 and 9 % at
+// 8192 x 8192, 3-6 % at 512-2048 points -- many short-lived workgroups keep more loads
+// in flight, and the grid's last round is one group deep.
 template <int LOG2N, int DIR, int TPW, bool SCALED>
 __global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT * TPW)
 k_fft_rows(const float2 *__restrict__ in, float2 *__restrict__ out, long batch,
@@ -109,68 +113,37 @@ k_fft_rows(const float2 *__restrict__ in, float2 *__restrict__ out, long batch,
     constexpr int N = G::N, E = G::E, TPT = G::TPT;
     constexpr int SLICE = G::uses_lds ? Lds<kLdsPad>::size(N) : 0;
     constexpr int LDS = G::uses_lds ? TPW * SLICE : 1;
-    static_assert(TPW > 1, "rows of 256 or more threads run k_fft_rows_one");
     __shared__ float2 lds[LDS];
-    float2 v[E], vn[E];
-    bool busy = false;
-    const int t = threadIdx.x;
-    const int w = t / TPT, tl = t % TPT;
-    const long groups = (batch + TPW - 1) / TPW;
-    const long last = batch - 1;
-    long grp = blockIdx.x;
-    if (grp < groups) {
-        const float2 *src = in + min(grp * TPW + w, last) * N + tl;
+    float2 v[E];
+    if constexpr (TPW == 1) {
+        // one row per workgroup, the grid one workgroup per row
+        const int tl = threadIdx.x;
+        const long row = blockIdx.x;
+        const float2 *src = in + row * N + tl;
 #pragma unroll
         for (int i = 0; i < E; ++i) v[i] = ld_stream(src + TPT * i);
-    }
-    for (; grp < groups; grp += gridDim.x) {  // uniform across the workgroup
-        const long row = grp * TPW + w;
-        const long next = grp + gridDim.x;
-        if (next < groups) {
-            const float2 *src = in + min(next * TPW + w, last) * N + tl;
+        block_fft<LOG2N, DIR, kLdsPad, false, kRowsTw<LOG2N>>(v, lds, tl, tw, false);
+        float2 *dst = out + row * N + tl;
 #pragma unroll
-            for (int i = 0; i < E; ++i) vn[i] = ld_stream(src + TPT * i);
-        }
+        for (int i = 0; i < E; ++i) st_stream(dst + TPT * i, SCALED ? cscale(v[i], scale) : v[i]);
+    } else {
+        const int t = threadIdx.x;
+        const int w = t / TPT, tl = t % TPT;
+        const long want = (long)blockIdx.x * TPW + w;
+        const long row = want < batch ? want : batch - 1;
+        const float2 *src = in + row * N + tl;
+#pragma unroll
+        for (int i = 0; i < E; ++i) v[i] = ld_stream(src + TPT * i);
         {
             const int to = opaque_lane(t);
-            block_fft<LOG2N, DIR, kLdsPad, false, kRowsTw<LOG2N>>(v, lds + (to / TPT) * SLICE, to % TPT, tw, busy);
+            block_fft<LOG2N, DIR, kLdsPad, false, kRowsTw<LOG2N>>(v, lds + (to / TPT) * SLICE, to % TPT, tw, false);
         }
-        busy = true;
-        if (row < batch) {
+        if (want < batch) {
             float2 *dst = out + row * N + tl;
 #pragma unroll
             for (int i = 0; i < E; ++i) st_stream(dst + TPT * i, SCALED ? cscale(v[i], scale) : v[i]);
         }
-#pragma unroll
-        for (int i = 0; i < E; ++i) v[i] = vn[i];
     }
-}
-
-// Rows of 4096 and 8192 points: one row per workgroup (TPT = 256 or 512 threads), one
-// workgroup per row, nontemporal loads and stores.  With twiddles by powers the kernel
-// holds 68 (4096) / 75 (8192) VGPRs, so LDS alone bounds the residency: four 4096-point
-// rows per CU (34 KiB each), two of 8192 (68 KiB).  Measured against the persistent form
-// above with the next row prefetched into registers (and the block table's twiddles kept
-// in registers across rows: 167 / 206 VGPRs, three and two waves per SIMD): 6.5 % faster
-// at 4096 x 16384 and 9 % at 8192 x 8192 -- many short-lived workgroups keep more loads
-// in flight, and the grid's last round is one row deep.
-template <int LOG2N, int DIR, bool SCALED>
-__global__ void __launch_bounds__(BlockGeom<LOG2N>::TPT)
-k_fft_rows_one(const float2 *__restrict__ in, float2 *__restrict__ out, const float2 *__restrict__ tw, float scale) {
-    using G = BlockGeom<LOG2N>;
-    constexpr int N = G::N, E = G::E, TPT = G::TPT;
-    static_assert(G::uses_lds && TPT >= 256, "one row per workgroup from 4096 points");
-    __shared__ float2 lds[Lds<kLdsPad>::size(N)];
-    float2 v[E];
-    const int tl = threadIdx.x;
-    const long row = blockIdx.x;
-    const float2 *src = in + row * N + tl;
-#pragma unroll
-    for (int i = 0; i < E; ++i) v[i] = ld_stream(src + TPT * i);
-    block_fft<LOG2N, DIR, kLdsPad, false, kRowsTw<LOG2N>>(v, lds, tl, tw, false);
-    float2 *dst = out + row * N + tl;
-#pragma unroll
-    for (int i = 0; i < E; ++i) st_stream(dst + TPT * i, SCALED ? cscale(v[i], scale) : v[i]);
 }
 
 // Rows on the reference's tile-planarized layout (PlannarizedComplexArray<5>): the same
@@ -230,26 +203,18 @@ static long resident_blocks(K kernel, int block_threads) {
     return co_resident_blocks(reinterpret_cast<const void *>(kernel), block_threads);
 }
 
-// persistent grid: this many resident rounds of workgroups
+// persistent grid of the planarized rows kernel: this many resident rounds of workgroups
 constexpr long kRowsGridRounds = 4;
 
 template <int LOG2N, int DIR>
 static hipError_t launch_rows_t(const float2 *in, float2 *out, long batch, const float2 *tw, float scale,
                                 hipStream_t st) {
     using G = BlockGeom<LOG2N>;
-    if constexpr (G::TPT >= 256) {
-        // one workgroup per row
-        if (batch > 0x7fffffffL) return hipErrorInvalidValue;
-        auto kernel = (scale != 1.0f) ? k_fft_rows_one<LOG2N, DIR, true> : k_fft_rows_one<LOG2N, DIR, false>;
-        hipLaunchKernelGGL(kernel, dim3((unsigned)batch), dim3(G::TPT), 0, st, in, out, tw, scale);
-    } else {
-        constexpr int TPW = 256 / G::TPT;
-        const long groups = (batch + TPW - 1) / TPW;
-        auto kernel = (scale != 1.0f) ? k_fft_rows<LOG2N, DIR, TPW, true> : k_fft_rows<LOG2N, DIR, TPW, false>;
-        long blocks = resident_blocks(k_fft_rows<LOG2N, DIR, TPW, false>, G::TPT * TPW) * kRowsGridRounds;
-        if (blocks > groups) blocks = groups;
-        hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(G::TPT * TPW), 0, st, in, out, batch, tw, scale);
-    }
+    constexpr int TPW = (G::TPT >= 256) ? 1 : 256 / G::TPT;
+    const long groups = (batch + TPW - 1) / TPW;  // one workgroup each
+    if (groups > 0x7fffffffL) return hipErrorInvalidValue;
+    auto kernel = (scale != 1.0f) ? k_fft_rows<LOG2N, DIR, TPW, true> : k_fft_rows<LOG2N, DIR, TPW, false>;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)groups), dim3(G::TPT * TPW), 0, st, in, out, batch, tw, scale);
     return hipGetLastError();
 }
 
