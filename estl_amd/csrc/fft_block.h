@@ -128,9 +128,10 @@ struct Lds<kLdsXor> {
 // All threads of the workgroup must call it together (it holds workgroup barriers).
 // If `lds_busy` is set, the slice is still being read on entry and a barrier is placed
 // before the first LDS write.  HOIST_TW lets the compiler keep the twiddles of every pass
-// in registers across calls made from a loop (fastest when the loop runs one size, as
-// in k_fft_rows); callers that dispatch many sizes in one loop pass false so each
-// size's twiddles are reloaded (from L1/L2) instead of all staying live.
+// in registers across calls made from a loop over one size; callers that dispatch many
+// sizes in one loop pass false so each size's twiddles are reloaded (from L1/L2)
+// instead of all staying live.  No kernel sets it at present: the rows kernels, its
+// last user, run faster at one row per workgroup with twiddles by powers (k_fft_rows).
 template <int LOG2N, int DIR, int LAYOUT = kLdsPad, bool HOIST_TW = false, int TWSRC = kTwBlock>
 __device__ __forceinline__ void block_fft(float2 (&v)[BlockGeom<LOG2N>::E], float2 *lds, int tl,
                                           const float2 *__restrict__ tw, bool lds_busy = false) {
