@@ -145,6 +145,22 @@ k_fft_rows_h(const f16x2 *__restrict__ in, f16x2 *__restrict__ out, long batch, 
     constexpr int SLICE = G::uses_lds ? Lds<kLdsPad>::size(N) : 0;
     constexpr int LDS = G::uses_lds ? TPW * SLICE : 1;
     __shared__ float2 lds[LDS];
+    if constexpr (TPW == 1) {
+        // one row per workgroup, the grid one workgroup per row (no clamp of the row
+        // index or store guard: 69 / 77 VGPRs at 4096 / 8192 points instead of 75 / 83,
+        // 1-5 % faster)
+        const int tl = threadIdx.x;
+        const long row = blockIdx.x;
+        float2 v[E];
+        const f16x2 *src = in + row * N + tl;
+#pragma unroll
+        for (int i = 0; i < E; ++i) v[i] = ld_elem(src + TPT * i);
+        block_fft<LOG2N, DIR, kLdsPad, false, kRowsTw<LOG2N>>(v, lds, tl, tw, false);
+        f16x2 *dst = out + row * N + tl;
+#pragma unroll
+        for (int i = 0; i < E; ++i) st_elem(dst + TPT * i, cscale(v[i], scale));
+        return;
+    }
     const long groups = (batch + TPW - 1) / TPW;
     bool busy = false;
     for (long grp = blockIdx.x; grp < groups; grp += gridDim.x) {  // uniform across the workgroup
