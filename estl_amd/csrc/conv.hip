@@ -155,7 +155,9 @@ __device__ __forceinline__ float ld_real(const _Float16 *p) { return (float)*p; 
 __device__ __forceinline__ void st_real(float *p, float v) { st_stream_f(p, v); }
 __device__ __forceinline__ void st_real(_Float16 *p, float v) { *p = (_Float16)v; }
 
-// Persistent over pairs of real rows (TPW pairs per workgroup), like k_fft_rows.  With
+// Persistent over pairs of real rows (TPW pairs per workgroup, four resident rounds of
+// workgroups; one workgroup per group measured within 2 % either way at 1024-8192
+// points, so the grid stays).  With
 // a request group, `rows` counts the group's pairs twice and each slot takes its pair's
 // member: input rows, spectrum and output rows.  R: the stored real type, float or
 // _Float16 (FP16 storage: the spectrum has been divided by the filter's L1 norm, so
